@@ -1,0 +1,239 @@
+"""Python host side of the backend: ctypes binding of the C ABI
+(include/capaz.h) and a ``Domain`` class whose methods carry the names of the
+reference entry points they stand in for (defines.h:115-224), so driver code
+and tests read like capaz's own ``time_step_function`` (main.cpp:93-108).
+
+The library is ``capaz_amd/libcapaz_amd.so`` (built in-tree by
+``capaz_amd/build.py``).  There is no fallback: if the library or a GPU is
+missing, creating a Domain raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libcapaz_amd.so")
+
+# Field names in ptr_Arrays member order (defines.h:65-76) == capaz_field_id.
+FIELDS = (
+    "S_n", "S_w", "P_w", "P_n", "ro_w", "ro_n", "ux_w", "uy_w", "uz_w", "ux_n", "uy_n", "uz_n",
+    "Xi_w", "Xi_n", "roS_w", "roS_w_old", "roS_n", "roS_n_old", "m", "K",
+    "S_g", "P_g", "ro_g", "ux_g", "uy_g", "uz_g", "Xi_g", "roS_g", "roS_g_old",
+    "T", "H_w", "H_n", "H_g", "H_r", "E", "E_new",
+)
+FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
+
+# The C entry points of include/capaz.h with their ctypes signatures.
+EXPORTS = {
+    "capaz_params_init": (None, ["params_p"]),
+    "capaz_abi_version": (ctypes.c_int, []),
+    "capaz_create": (ctypes.c_int, ["params_p", "ctx_pp"]),
+    "capaz_destroy": (ctypes.c_int, ["ctx"]),
+    "capaz_last_error": (ctypes.c_char_p, ["ctx"]),
+    "capaz_cells": (ctypes.c_size_t, ["ctx"]),
+    "capaz_has_field": (ctypes.c_int, ["ctx", ctypes.c_int]),
+    "capaz_set_field": (ctypes.c_int, ["ctx", ctypes.c_int, "dbl_p", ctypes.c_size_t]),
+    "capaz_get_field": (ctypes.c_int, ["ctx", ctypes.c_int, "dbl_p", ctypes.c_size_t]),
+    "capaz_field_device_ptr": (ctypes.c_int, ["ctx", ctypes.c_int, "dbl_pp"]),
+    "capaz_set_field_device": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t]),
+    "capaz_S": (ctypes.c_int, ["ctx"]),
+    "capaz_P_Xi_ro": (ctypes.c_int, ["ctx"]),
+    "capaz_u": (ctypes.c_int, ["ctx"]),
+    "capaz_roS": (ctypes.c_int, ["ctx", ctypes.c_double]),
+    "capaz_energy_HE": (ctypes.c_int, ["ctx"]),
+    "capaz_energy_E": (ctypes.c_int, ["ctx"]),
+    "capaz_newton": (ctypes.c_int, ["ctx"]),
+    "capaz_boundary": (ctypes.c_int, ["ctx"]),
+    "capaz_step_sequence": (ctypes.c_int, ["ctx", ctypes.c_double]),
+    "capaz_step": (ctypes.c_int, ["ctx", ctypes.c_long, ctypes.c_long]),
+    "capaz_sync": (ctypes.c_int, ["ctx"]),
+    "capaz_stream": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_void_p)]),
+    "capaz_load_face": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "capaz_save_face": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+}
+
+
+class Params(ctypes.Structure):
+    """capaz_params (include/capaz.h)."""
+    _fields_ = [
+        ("struct_size", ctypes.c_uint), ("model", ctypes.c_int), ("energy", ctypes.c_int),
+        ("closure", ctypes.c_int), ("Nx", ctypes.c_int), ("Ny", ctypes.c_int), ("Nz", ctypes.c_int),
+        ("source", ctypes.c_int), ("newton_iterations", ctypes.c_int), ("device", ctypes.c_int),
+        ("hx", ctypes.c_double), ("hy", ctypes.c_double), ("hz", ctypes.c_double), ("dt", ctypes.c_double),
+        ("P_atm", ctypes.c_double), ("g_const", ctypes.c_double),
+        ("mu_w", ctypes.c_double), ("mu_n", ctypes.c_double), ("mu_g", ctypes.c_double),
+        ("ro0_w", ctypes.c_double), ("ro0_n", ctypes.c_double), ("ro0_g", ctypes.c_double),
+        ("beta_w", ctypes.c_double), ("beta_n", ctypes.c_double), ("lambda_", ctypes.c_double),
+        ("S_wr", ctypes.c_double), ("S_nr", ctypes.c_double), ("S_gr", ctypes.c_double),
+        ("P_d", ctypes.c_double), ("S_w_gr", ctypes.c_double), ("S_n_gr", ctypes.c_double),
+        ("corey_n", ctypes.c_double),
+    ]
+
+
+_TYPES = {
+    "params_p": ctypes.POINTER(Params),
+    "ctx": ctypes.c_void_p,
+    "ctx_pp": ctypes.POINTER(ctypes.c_void_p),
+    "dbl_p": ctypes.POINTER(ctypes.c_double),
+    "dbl_pp": ctypes.POINTER(ctypes.POINTER(ctypes.c_double)),
+}
+
+_lib = None
+
+
+class CapazError(RuntimeError):
+    pass
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def load() -> ctypes.CDLL:
+    """Loads libcapaz_amd.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CapazError(f"{LIB_PATH} is missing: build it with `python -m capaz_amd.build` "
+                             "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = [_TYPES.get(a, a) if isinstance(a, str) else a for a in args]
+        _lib = L
+    return _lib
+
+
+def params_from_keys(keys: dict, device: int = 0) -> Params:
+    """capaz_params from defines.ini keys (main.cpp:1327-1626) plus MODEL,
+    ENERGY, CLOSURE ("reference"/"corey") and COREY_N."""
+    p = Params()
+    p.struct_size = ctypes.sizeof(Params)
+    g = lambda k, d=0.0: float(keys.get(k, d))
+    p.model = int(keys.get("MODEL", 3))
+    p.energy = int(keys.get("ENERGY", 0))
+    p.closure = 1 if keys.get("CLOSURE", "reference") == "corey" else 0
+    p.Nx, p.Ny, p.Nz = int(keys["NX"]), int(keys["NY"]), int(keys["NZ"])
+    p.source = int(keys.get("SOURCE", 0))
+    p.newton_iterations = int(keys.get("ITERATIONS", 1))
+    p.device = device
+    p.hx, p.hy, p.hz, p.dt = g("HX"), g("HY"), g("HZ"), g("DT")
+    p.P_atm, p.g_const = g("P_ATM"), g("G_CONST")
+    p.mu_w, p.mu_n, p.mu_g = g("MU_W"), g("MU_N"), g("MU_G")
+    p.ro0_w, p.ro0_n, p.ro0_g = g("RO_W"), g("RO_N"), g("RO_G")
+    p.beta_w, p.beta_n = g("BETA_W"), g("BETA_N")
+    p.lambda_ = g("LAMBDA_0")
+    p.S_wr, p.S_nr, p.S_gr = g("S_WR_0"), g("S_NR_0"), g("S_GR_0")
+    p.P_d = g("P_D_0")
+    p.S_w_gr, p.S_n_gr = g("S_W_GR"), g("S_N_GR")
+    p.corey_n = g("COREY_N", 2.0)
+    return p
+
+
+class Domain:
+    """One single-GPU domain: a capaz_ctx and every field of its model."""
+
+    def __init__(self, keys: dict, fields: dict[str, np.ndarray] | None = None, device: int = 0):
+        L = load()
+        self.keys = dict(keys)
+        self.params = params_from_keys(keys, device)
+        self.shape = (self.params.Nz, self.params.Ny, self.params.Nx)
+        self.n = int(np.prod(self.shape))
+        ctx = ctypes.c_void_p()
+        rc = L.capaz_create(ctypes.byref(self.params), ctypes.byref(ctx))
+        if rc != 0:
+            raise CapazError(f"capaz_create failed ({rc}): {L.capaz_last_error(None).decode()}")
+        self._ctx = ctx
+        self.names = [f for f in FIELDS if L.capaz_has_field(ctx, FIELD_ID[f])]
+        if fields:
+            for name, v in fields.items():
+                self.set_field(name, v)
+
+    # -- errors -------------------------------------------------------------
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            msg = load().capaz_last_error(self._ctx).decode()
+            raise CapazError(f"{what} failed ({rc}): {msg}")
+
+    # -- data (load_data_to_device / load_data_to_host) ---------------------
+    def set_field(self, name: str, value) -> None:
+        a = np.ascontiguousarray(np.broadcast_to(np.asarray(value, dtype=np.float64), self.shape))
+        self._check(load().capaz_set_field(self._ctx, FIELD_ID[name],
+                                           a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), self.n),
+                    f"capaz_set_field({name})")
+
+    def field(self, name: str) -> np.ndarray:
+        out = np.empty(self.shape)
+        self._check(load().capaz_get_field(self._ctx, FIELD_ID[name],
+                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), self.n),
+                    f"capaz_get_field({name})")
+        return out
+
+    def fields(self) -> dict[str, np.ndarray]:
+        return {n: self.field(n) for n in self.names}
+
+    def device_ptr(self, name: str) -> int:
+        p = ctypes.POINTER(ctypes.c_double)()
+        self._check(load().capaz_field_device_ptr(self._ctx, FIELD_ID[name], ctypes.byref(p)),
+                    "capaz_field_device_ptr")
+        return ctypes.cast(p, ctypes.c_void_p).value or 0
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        self._check(load().capaz_stream(self._ctx, ctypes.byref(s)), "capaz_stream")
+        return s.value or 0
+
+    # -- entry points (defines.h:186-193, 220-222) ----------------------------
+    def S_calculation(self):
+        self._check(load().capaz_S(self._ctx), "capaz_S")
+
+    def ro_P_Xi_calculation(self):
+        self._check(load().capaz_P_Xi_ro(self._ctx), "capaz_P_Xi_ro")
+
+    def u_calculation(self):
+        self._check(load().capaz_u(self._ctx), "capaz_u")
+
+    def roS_calculation(self, t: float = 0.0):
+        self._check(load().capaz_roS(self._ctx, t), "capaz_roS")
+
+    def H_E_current_calculation(self):
+        self._check(load().capaz_energy_HE(self._ctx), "capaz_energy_HE")
+
+    def E_calculation(self):
+        self._check(load().capaz_energy_E(self._ctx), "capaz_energy_E")
+
+    def P_S_calculation(self):
+        self._check(load().capaz_newton(self._ctx), "capaz_newton")
+
+    def boundary_conditions(self):
+        self._check(load().capaz_boundary(self._ctx), "capaz_boundary")
+
+    def time_step_function(self, t: float):
+        self._check(load().capaz_step_sequence(self._ctx, t), "capaz_step_sequence")
+
+    def call(self, entry: str, t: float = 0.0) -> None:
+        fn = getattr(self, entry)
+        fn(t) if entry in ("roS_calculation", "time_step_function") else fn()
+
+    # -- fused path ----------------------------------------------------------
+    def run(self, nsteps: int, first_step: int = 1) -> None:
+        """nsteps fused time steps (capaz_step)."""
+        self._check(load().capaz_step(self._ctx, first_step, nsteps), "capaz_step")
+
+    def sync(self) -> None:
+        self._check(load().capaz_sync(self._ctx), "capaz_sync")
+
+    def close(self) -> None:
+        if getattr(self, "_ctx", None):
+            load().capaz_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

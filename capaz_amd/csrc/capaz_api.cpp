@@ -1,0 +1,365 @@
+// C ABI of the backend (include/capaz.h): context, device memory of every
+// field, the entry-point sequence and the fused step.
+//
+// State model.  Every field of the model has one device array, the current
+// one, on which the entry-point kernels work in place exactly as the
+// reference's loops work on its host arrays.  The primary state (P_w, S_n,
+// and for three phases S_w, P_n, P_g) has a second array: a fused step reads
+// the current arrays and writes the second ones, then the two swap.  A fused
+// step does not store the intermediate fields (S_g / S_w, Xi, ro, u, roS, the
+// interior P_n / P_g); they are marked stale and recomputed from the previous
+// state -- which the second arrays still hold -- before anything reads them,
+// so every field always reads as after the same number of reference steps.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <utility>
+
+#include "../../include/capaz.h"
+#include "capaz_derive.h"
+#include "capaz_launch.h"
+
+using capaz::DevFields;
+using capaz::DevParams;
+
+struct capaz_ctx {
+    capaz_params user;
+    DevParams p;
+    hipStream_t stream = nullptr;
+    double* cur[capaz::F_COUNT] = {};
+    double* alt[capaz::F_COUNT] = {};
+    bool primary[capaz::F_COUNT] = {};
+    bool stale = false;     // intermediates belong to an unrun materialisation
+    bool m_checked = false; // m has been scanned for zeros since it last changed
+    bool m_has_zero = false;
+    int* d_flag = nullptr;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(capaz_ctx* c, int code, const std::string& msg) {
+    if (c)
+        c->err = msg;
+    else
+        g_create_error = msg;
+    return code;
+}
+
+int hip_fail(capaz_ctx* c, hipError_t e, const char* what) {
+    return fail(c, e == hipErrorOutOfMemory ? CAPAZ_ERR_NOMEM : CAPAZ_ERR_DEVICE,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define CAPAZ_HIP(c, call)                                   \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return hip_fail((c), e_, #call); \
+    } while (0)
+
+bool field_exists(const capaz_params& u, int f) {
+    if (f < 0 || f >= capaz::F_COUNT) return false;
+    if (f >= capaz::F_T) return u.energy == 1;
+    if (f >= capaz::F_S_g) return u.model == 3;
+    return true;
+}
+
+// Fields whose value carries over from step to step (the rest are recomputed
+// from these at every step).
+bool is_primary(const capaz_params& u, int f) {
+    using namespace capaz;
+    if (f == F_P_w || f == F_S_n) return true;
+    if (u.model == 3 && (f == F_S_w || f == F_P_n || f == F_P_g)) return true;
+    if (u.energy && f == F_T) return true;
+    return false;
+}
+
+DevFields fields_of(capaz_ctx* c) {
+    DevFields F;
+    for (int f = 0; f < capaz::F_COUNT; ++f) F.f[f] = c->cur[f];
+    return F;
+}
+
+int run_entry(capaz_ctx* c, capaz::Entry e, DevFields F) {
+    CAPAZ_HIP(c, capaz::launch_entry(e, c->p, F, c->stream));
+    return CAPAZ_OK;
+}
+
+// Recomputes the intermediate fields of the last fused step from the
+// previous primary state (held in the second arrays).
+int materialize(capaz_ctx* c) {
+    if (!c->stale) return CAPAZ_OK;
+    DevFields F = fields_of(c);
+    for (int f = 0; f < capaz::F_COUNT; ++f)
+        if (c->primary[f]) F.f[f] = c->alt[f];
+    int rc;
+    if ((rc = run_entry(c, capaz::Entry::S, F))) return rc;
+    if ((rc = run_entry(c, capaz::Entry::P_Xi_ro, F))) return rc;
+    if ((rc = run_entry(c, capaz::Entry::U, F))) return rc;
+    if ((rc = run_entry(c, capaz::Entry::RoS, F))) return rc;
+    if (c->user.model == 3) {
+        CAPAZ_HIP(c, capaz::launch_copy_interior(c->p, c->alt[capaz::F_P_n], c->cur[capaz::F_P_n], c->stream));
+        CAPAZ_HIP(c, capaz::launch_copy_interior(c->p, c->alt[capaz::F_P_g], c->cur[capaz::F_P_g], c->stream));
+    }
+    c->stale = false;
+    return CAPAZ_OK;
+}
+
+int check_ctx(capaz_ctx* c) {
+    if (!c) return CAPAZ_ERR_INVALID;
+    CAPAZ_HIP(c, hipSetDevice(c->user.device));
+    return CAPAZ_OK;
+}
+
+int check_field(capaz_ctx* c, int field, size_t n) {
+    if (!field_exists(c->user, field)) return fail(c, CAPAZ_ERR_INVALID, "field does not exist for this model");
+    if (n != (size_t)c->p.n) return fail(c, CAPAZ_ERR_INVALID, "n must equal Nx*Ny*Nz");
+    return CAPAZ_OK;
+}
+
+int scan_m(capaz_ctx* c) {
+    if (c->m_checked) return CAPAZ_OK;
+    CAPAZ_HIP(c, hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
+    CAPAZ_HIP(c, capaz::launch_m_zero(c->p, c->cur[capaz::F_m], c->d_flag, c->stream));
+    int h = 0;
+    CAPAZ_HIP(c, hipMemcpyAsync(&h, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    CAPAZ_HIP(c, hipStreamSynchronize(c->stream));
+    c->m_has_zero = h != 0;
+    c->m_checked = true;
+    return CAPAZ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void capaz_params_init(capaz_params* p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->struct_size = sizeof(capaz_params);
+    p->model = CAPAZ_THREE_PHASE;
+    p->newton_iterations = 1;
+    p->corey_n = 2.;
+}
+
+int capaz_abi_version(void) { return CAPAZ_ABI_VERSION; }
+
+int capaz_create(const capaz_params* u, capaz_ctx** out) {
+    if (!u || !out) return fail(nullptr, CAPAZ_ERR_INVALID, "NULL argument");
+    *out = nullptr;
+    std::string why = capaz::validate(*u);
+    if (!why.empty()) return fail(nullptr, CAPAZ_ERR_INVALID, why);
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return fail(nullptr, CAPAZ_ERR_DEVICE, "no HIP device available");
+    if (u->device < 0 || u->device >= ndev) return fail(nullptr, CAPAZ_ERR_INVALID, "device ordinal out of range");
+    capaz_ctx* c = new (std::nothrow) capaz_ctx;
+    if (!c) return fail(nullptr, CAPAZ_ERR_NOMEM, "host allocation failed");
+    c->user = *u;
+    c->p = capaz::derive(*u);
+    auto bail = [&](hipError_t err, const char* what) {
+        g_create_error = std::string(what) + ": " + hipGetErrorString(err);
+        capaz_destroy(c);
+        return err == hipErrorOutOfMemory ? CAPAZ_ERR_NOMEM : CAPAZ_ERR_DEVICE;
+    };
+    if ((e = hipSetDevice(u->device)) != hipSuccess) return bail(e, "hipSetDevice");
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bail(e, "hipStreamCreate");
+    if ((e = hipMalloc(&c->d_flag, sizeof(int))) != hipSuccess) return bail(e, "hipMalloc");
+    const size_t bytes = (size_t)c->p.n * sizeof(double);
+    for (int f = 0; f < capaz::F_COUNT; ++f) {
+        if (!field_exists(*u, f)) continue;
+        if ((e = hipMalloc(&c->cur[f], bytes)) != hipSuccess) return bail(e, "hipMalloc");
+        if ((e = hipMemsetAsync(c->cur[f], 0, bytes, c->stream)) != hipSuccess) return bail(e, "hipMemset");
+        c->primary[f] = is_primary(*u, f) && !u->energy;
+        if (c->primary[f]) {
+            if ((e = hipMalloc(&c->alt[f], bytes)) != hipSuccess) return bail(e, "hipMalloc");
+            if ((e = hipMemsetAsync(c->alt[f], 0, bytes, c->stream)) != hipSuccess) return bail(e, "hipMemset");
+        }
+    }
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail(e, "hipStreamSynchronize");
+    *out = c;
+    return CAPAZ_OK;
+}
+
+int capaz_destroy(capaz_ctx* c) {
+    if (!c) return CAPAZ_OK;
+    (void)hipSetDevice(c->user.device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (int f = 0; f < capaz::F_COUNT; ++f) {
+        if (c->cur[f]) (void)hipFree(c->cur[f]);
+        if (c->alt[f]) (void)hipFree(c->alt[f]);
+    }
+    if (c->d_flag) (void)hipFree(c->d_flag);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CAPAZ_OK;
+}
+
+const char* capaz_last_error(const capaz_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+size_t capaz_cells(const capaz_ctx* c) { return c ? (size_t)c->p.n : 0; }
+
+int capaz_has_field(const capaz_ctx* c, int field) { return c && field_exists(c->user, field) ? 1 : 0; }
+
+int capaz_set_field(capaz_ctx* c, int field, const double* host, size_t n) {
+    int rc;
+    if ((rc = check_ctx(c))) return rc;
+    if ((rc = check_field(c, field, n))) return rc;
+    if (!host) return fail(c, CAPAZ_ERR_INVALID, "NULL host pointer");
+    if ((rc = materialize(c))) return rc;
+    CAPAZ_HIP(c, hipMemcpyAsync(c->cur[field], host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    CAPAZ_HIP(c, hipStreamSynchronize(c->stream));
+    if (field == capaz::F_m) c->m_checked = false;
+    return CAPAZ_OK;
+}
+
+int capaz_set_field_device(capaz_ctx* c, int field, const double* dsrc, size_t n) {
+    int rc;
+    if ((rc = check_ctx(c))) return rc;
+    if ((rc = check_field(c, field, n))) return rc;
+    if (!dsrc) return fail(c, CAPAZ_ERR_INVALID, "NULL device pointer");
+    if ((rc = materialize(c))) return rc;
+    CAPAZ_HIP(c, hipMemcpyAsync(c->cur[field], dsrc, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (field == capaz::F_m) c->m_checked = false;
+    return CAPAZ_OK;
+}
+
+int capaz_get_field(capaz_ctx* c, int field, double* host, size_t n) {
+    int rc;
+    if ((rc = check_ctx(c))) return rc;
+    if ((rc = check_field(c, field, n))) return rc;
+    if (!host) return fail(c, CAPAZ_ERR_INVALID, "NULL host pointer");
+    if ((rc = materialize(c))) return rc;
+    CAPAZ_HIP(c, hipMemcpyAsync(host, c->cur[field], n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CAPAZ_HIP(c, hipStreamSynchronize(c->stream));
+    return CAPAZ_OK;
+}
+
+int capaz_field_device_ptr(capaz_ctx* c, int field, double** dptr) {
+    int rc;
+    if ((rc = check_ctx(c))) return rc;
+    if (!field_exists(c->user, field)) return fail(c, CAPAZ_ERR_INVALID, "field does not exist for this model");
+    if (!dptr) return fail(c, CAPAZ_ERR_INVALID, "NULL dptr");
+    if ((rc = materialize(c))) return rc;
+    *dptr = c->cur[field];
+    if (field == capaz::F_m) c->m_checked = false;  // the caller may write through the pointer
+    return CAPAZ_OK;
+}
+
+#define CAPAZ_ENTRY(name, entry, guard)                                          \
+    int name(capaz_ctx* c) {                                                     \
+        int rc;                                                                  \
+        if ((rc = check_ctx(c))) return rc;                                      \
+        if (!(guard)) return fail(c, CAPAZ_ERR_STATE, #name ": not in this model"); \
+        if ((rc = materialize(c))) return rc;                                    \
+        return run_entry(c, capaz::Entry::entry, fields_of(c));                  \
+    }
+
+CAPAZ_ENTRY(capaz_S, S, true)
+CAPAZ_ENTRY(capaz_P_Xi_ro, P_Xi_ro, true)
+CAPAZ_ENTRY(capaz_u, U, true)
+CAPAZ_ENTRY(capaz_energy_HE, HE, c->user.energy)
+CAPAZ_ENTRY(capaz_energy_E, E, c->user.energy)
+CAPAZ_ENTRY(capaz_newton, Newton, true)
+CAPAZ_ENTRY(capaz_boundary, Boundary, true)
+
+int capaz_roS(capaz_ctx* c, double t) {
+    (void)t;  // assign_roS_nr does not read t (cpu.cpp:619-731)
+    int rc;
+    if ((rc = check_ctx(c))) return rc;
+    if ((rc = materialize(c))) return rc;
+    return run_entry(c, capaz::Entry::RoS, fields_of(c));
+}
+
+int capaz_step_sequence(capaz_ctx* c, double t) {
+    int rc;
+    if ((rc = capaz_S(c))) return rc;
+    if ((rc = capaz_P_Xi_ro(c))) return rc;
+    if ((rc = capaz_u(c))) return rc;
+    if ((rc = capaz_roS(c, t))) return rc;
+    if (c->user.energy) {
+        if ((rc = capaz_energy_HE(c))) return rc;
+        if ((rc = capaz_energy_E(c))) return rc;
+    }
+    if ((rc = capaz_newton(c))) return rc;
+    return capaz_boundary(c);
+}
+
+int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
+    int rc;
+    if ((rc = check_ctx(c))) return rc;
+    if (nsteps < 0) return fail(c, CAPAZ_ERR_INVALID, "nsteps must be >= 0");
+    if ((rc = scan_m(c))) return rc;
+    if (c->user.energy || c->m_has_zero) {
+        // No fused kernel for the energy model; interior cells with m == 0 keep
+        // the reference's stale-roS rule, which only the sequence reproduces.
+        for (long s = first_step; s < first_step + nsteps; ++s)
+            if ((rc = capaz_step_sequence(c, (double)s * c->user.dt))) return rc;
+        return CAPAZ_OK;
+    }
+    using namespace capaz;
+    for (long s = 0; s < nsteps; ++s) {
+        StepIO io;
+        io.Pw = c->cur[F_P_w];
+        io.Sw = c->cur[F_S_w];
+        io.Sn = c->cur[F_S_n];
+        io.Pn = c->cur[F_P_n];
+        io.Pg = c->cur[F_P_g];
+        io.m = c->cur[F_m];
+        io.K = c->cur[F_K];
+        io.Pw1 = c->alt[F_P_w];
+        io.Sw1 = c->alt[F_S_w];
+        io.Sn1 = c->alt[F_S_n];
+        io.Pn1 = c->alt[F_P_n];
+        io.Pg1 = c->alt[F_P_g];
+        CAPAZ_HIP(c, launch_fused_step(c->p, io, c->stream));
+        for (int f = 0; f < F_COUNT; ++f)
+            if (c->primary[f]) std::swap(c->cur[f], c->alt[f]);
+        c->stale = true;
+    }
+    return CAPAZ_OK;
+}
+
+int capaz_sync(capaz_ctx* c) {
+    int rc;
+    if ((rc = check_ctx(c))) return rc;
+    CAPAZ_HIP(c, hipStreamSynchronize(c->stream));
+    return CAPAZ_OK;
+}
+
+int capaz_stream(capaz_ctx* c, void** s) {
+    if (!c || !s) return CAPAZ_ERR_INVALID;
+    *s = (void*)c->stream;
+    return CAPAZ_OK;
+}
+
+static int face_call(capaz_ctx* c, int field, int axis, int side, double* buf, int save) {
+    int rc;
+    if ((rc = check_ctx(c))) return rc;
+    if (!field_exists(c->user, field)) return fail(c, CAPAZ_ERR_INVALID, "field does not exist for this model");
+    if (axis < 0 || axis > 2 || (side != 0 && side != 1) || !buf)
+        return fail(c, CAPAZ_ERR_INVALID, "axis must be 0..2, side 0 or 1, buffer non-NULL");
+    int n = axis == 0 ? c->p.Nx : (axis == 1 ? c->p.Ny : c->p.Nz);
+    if (n < 2) return fail(c, CAPAZ_ERR_INVALID, "axis has fewer than 2 planes");
+    if ((rc = materialize(c))) return rc;
+    // load_*: second / second-to-last plane; save_*: first / last plane
+    int pos = save ? (side == 0 ? 0 : n - 1) : (side == 0 ? 1 : n - 2);
+    CAPAZ_HIP(c, capaz::launch_face(c->p, c->cur[field], buf, axis, pos, save, c->stream));
+    if (save && field == capaz::F_m) c->m_checked = false;
+    return CAPAZ_OK;
+}
+
+int capaz_load_face(capaz_ctx* c, int field, int axis, int side, double* dbuf) {
+    return face_call(c, field, axis, side, dbuf, 0);
+}
+
+int capaz_save_face(capaz_ctx* c, int field, int axis, int side, const double* dbuf) {
+    return face_call(c, field, axis, side, const_cast<double*>(dbuf), 1);
+}
+
+}  // extern "C"

@@ -1,0 +1,420 @@
+// Per-cell arithmetic of the filtration step, written once for the HIP
+// kernels and for a host build of the same code (used to check the kernels'
+// arithmetic against the oracle on the CPU).
+//
+// Parity rule: every function below produces the same IEEE bits as the
+// reference CPU path (cpu.cpp, Two-phase/two-phase.cpp,
+// Three-phase/three-phase.cpp, gauss.cpp, energy.cpp) for finite operands in
+// the normal range, up to the sign of an exact zero.  Three transformations
+// are used, each exact:
+//   * a / b with a constant b is exact_div(a, b, RN(1/b)): one FMA residual
+//     correction of a*RN(1/b) returns RN(a/b) (Markstein's theorem, Muller et
+//     al., Handbook of Floating-Point Arithmetic, 2nd ed., Thm 4.10);
+//   * x / 2. is x * 0.5 (a power-of-two scaling is exact);
+//   * terms that are exact zeros in the reference (the CPU path's capillary
+//     pressures and well rates, three-phase.cpp:23-143,529-550) are folded:
+//     x + 0 == x and x - (+-0) == x for x != 0.
+// The code is built with -ffp-contract=off so no other FMA is formed.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+
+#include "capaz_params.h"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define CAPAZ_HD __host__ __device__ __forceinline__
+#else
+#define CAPAZ_HD inline
+#endif
+
+namespace capaz {
+
+// ---- arithmetic --------------------------------------------------------------
+
+// RN(a / b) given rb = RN(1 / b): q = RN(a*rb) is within one ulp of a/b, the
+// residual a - b*q is exact in an FMA, and one correction rounds correctly.
+CAPAZ_HD double exact_div(double a, double b, double rb) {
+    double q = a * rb;
+    double r = fma(-q, b, a);
+    return fma(r, rb, q);
+}
+
+CAPAZ_HD double half(double x) { return x * 0.5; }
+CAPAZ_HD double clamp01(double x) { return x < 0. ? 0. : (x > 1. ? 1. : x); }
+
+// ---- grid predicates ------------------------------------------------------------
+
+// assign_roS_nr's interior test (cpu.cpp:621), shared by the three-phase
+// assign_P_Xi/Newton (three-phase.cpp:225,288) and energy.cpp:237,272,330,372.
+CAPAZ_HD bool interior3(const DevParams& p, int i, int j, int k) {
+    return (((i != 0) && (i != p.Nx - 1)) || (p.Nx < 2)) && (j != 0) && (j != p.Ny - 1) &&
+           (((k != 0) && (k != p.Nz - 1)) || (p.Nz < 2));
+}
+
+// The two-phase Newton's interior test (two-phase.cpp:131).
+CAPAZ_HD bool interior_newton2(const DevParams& p, int i, int j, int k) {
+    return (i != 0) && (i != p.Nx - 1) && (j != 0) && (j != p.Ny - 1) &&
+           (((k != 0) && (k != p.Nz - 1)) || (p.Nz < 2));
+}
+
+// Border_S/Border_P's boundary test (two-phase.cpp:165; three-phase.cpp:347).
+CAPAZ_HD bool is_boundary(const DevParams& p, int i, int j, int k) {
+    return p.model == 2 ? !interior_newton2(p, i, j, k) : !interior3(p, i, j, k);
+}
+
+// set_boundary_basic_coordinate (cpu.cpp:107-141).
+CAPAZ_HD int64_t inward(const DevParams& p, int i, int j, int k) {
+    int i1 = i, j1 = j, k1 = k;
+    if (i == 0 && p.Nx > 2) i1++;
+    if (i == p.Nx - 1 && p.Nx > 2) i1--;
+    if (j == 0) j1++;
+    if (j == p.Ny - 1) j1--;
+    if (k == 0 && p.Nz > 2) k1++;
+    if (k == p.Nz - 1 && p.Nz > 2) k1--;
+    return (int64_t)i1 + (int64_t)j1 * p.sy + (int64_t)k1 * p.sz;
+}
+
+// ---- closures ---------------------------------------------------------------------
+
+CAPAZ_HD double corey_kr(const DevParams& p, double s) {
+    s = clamp01(s);
+    if (p.corey_n == 2.) return s * s;
+    return pow(s, p.corey_n);
+}
+
+// van Genuchten-type k_r (three-phase.cpp:146-186); p.vg_a = A/(A-1.),
+// p.vg_b = (A-1.)/A, p.vg_c = 2.*(A-1.)/A.
+CAPAZ_HD double vg_k_w(const DevParams& p, double Swe) {
+    if (Swe >= 1e-3) {
+        double t = 1. - pow(1. - pow(Swe, p.vg_a), p.vg_b);
+        return pow(Swe, 0.5) * pow(t, 2.);
+    }
+    return 0;
+}
+CAPAZ_HD double vg_k_g(const DevParams& p, double Sge) {
+    if (Sge >= 1e-3) return pow(Sge, 0.5) * pow(1. - pow(1. - Sge, p.vg_a), p.vg_c);
+    return 0;
+}
+CAPAZ_HD double vg_k_n(const DevParams& p, double Swe, double Sne) {
+    double Sge = 1. - Swe - Sne;
+    if (Sne >= 1e-3) {
+        double knw = pow(1. - Swe, 0.5) * pow(1. - pow(Swe, p.vg_a), p.vg_c);
+        double kng = pow(Sne, 0.5) * pow(1. - pow(1. - pow(Sne, p.vg_a), p.vg_b), 2.);
+        return Sne * knw * kng / (1 - Swe) / (1 - Sge);
+    }
+    return 0;
+}
+
+// Three-phase relative permeabilities k_w, k_n, k_g of a cell
+// (three-phase.cpp:202-208).
+CAPAZ_HD void relperm3(const DevParams& p, double S_w, double S_n, double& kw, double& kn, double& kg) {
+    double Swe = exact_div(S_w - p.S_wr, p.den3, p.r_den3);
+    double Sne = exact_div(S_n - p.S_nr, p.den3, p.r_den3);
+    double Sge = 1. - Swe - Sne;
+    if (p.closure == 1) {
+        kw = corey_kr(p, Swe);
+        kn = corey_kr(p, Sne);
+        kg = corey_kr(p, Sge);
+    } else {
+        kw = vg_k_w(p, Swe);
+        kg = vg_k_g(p, Sge);
+        kn = vg_k_n(p, Swe, Sne);
+    }
+}
+
+// Two-phase k_w, k_n and P_k (two-phase.cpp:6-12; Corey: no P_k).
+CAPAZ_HD void relperm2(const DevParams& p, double S_n, double& kw, double& kn, double& Pk) {
+    if (p.closure == 1) {
+        double se = exact_div(1. - S_n - p.S_wr, p.den2c, p.r_den2c);
+        kw = corey_kr(p, se);
+        kn = corey_kr(p, 1. - se);
+        Pk = 0.;
+    } else {
+        double se = exact_div(1. - S_n - p.S_wr, p.den2, p.r_den2);
+        kw = pow(se, p.bc_ew);
+        kn = (1. - se) * (1. - se) * (1 - pow(se, p.bc_en));
+        Pk = p.P_d * pow(se, p.bc_epk);
+    }
+}
+
+// Energy-model viscosities mu(T) (three-phase.cpp:212-214).
+CAPAZ_HD void mu_of_T(double T, double& mw, double& mn, double& mg) {
+    mw = 1. / (29.21 * T - 7506.64);
+    mn = 7.256E-10 * exp(4141.9 / T);
+    mg = 1.717E-5 * pow(exact_div(T, 273., 1. / 273.), 0.683);
+}
+
+// Xi = -1. * K * k / mu (two-phase.cpp:16-17, three-phase.cpp:216-222).
+CAPAZ_HD double xi_const(double K, double kr, double mu, double r_mu) { return exact_div((-1.) * K * kr, mu, r_mu); }
+CAPAZ_HD double xi_var(double K, double kr, double mu) { return (-1.) * K * kr / mu; }
+
+// Densities (cpu.cpp:143-169).
+CAPAZ_HD double ro_w_of(const DevParams& p, double P) { return p.ro0_w * (1. + p.beta_w * (P - p.P_atm)); }
+CAPAZ_HD double ro_n_of(const DevParams& p, double P) { return p.ro0_n * (1. + p.beta_n * (P - p.P_atm)); }
+CAPAZ_HD double ro_g_of(const DevParams& p, double P) { return exact_div(p.ro0_g * P, p.P_atm, p.r_P_atm); }
+
+// Energy-model densities (cpu.cpp:150-156, energy.cpp:146-170); T_0 = 273.
+constexpr double kT0 = 273.;
+constexpr double kAlfaW = 1.32E-7, kAlfaN = 9.2E-7;
+CAPAZ_HD double ro_w_T(const DevParams& p, double P, double T) {
+    return p.ro0_w * (1. + p.beta_w * (P - p.P_atm) - kAlfaW * (T - kT0));
+}
+CAPAZ_HD double ro_n_T(const DevParams& p, double P, double T) {
+    return p.ro0_n * (1. + p.beta_n * (P - p.P_atm) - kAlfaN * (T - kT0));
+}
+CAPAZ_HD double ro_g_T(const DevParams& p, double P, double T) {
+    return p.ro0_g * exact_div(P, p.P_atm, p.r_P_atm) * (kT0 / T);
+}
+
+// ---- Darcy flux (cpu.cpp:236-264, 619-731) -----------------------------------------
+
+// directed_difference: x1, x2 are the left/right driving forces, (Xi, ro) of
+// the cell (c) and of its minus (m) and plus (p) neighbours.
+CAPAZ_HD double directed(double x1, double x2, double Xic, double roc, double Xim, double rom, double Xip, double rop,
+                         double h, double rh) {
+    double t1 = (half(x2 + fabs(x2)) - half(x1 - fabs(x1))) * (-1.) * Xic * roc;
+    double t2 = half(x1 + fabs(x1)) * (-1.) * Xim * rom;
+    double t3 = half(x2 - fabs(x2)) * (-1.) * Xip * rop;
+    return exact_div(t1 - t2 + t3, h, rh);
+}
+
+// The upwind flux of one phase along one axis, from the face differences
+// dm = (P_c - P_m)/h and dp = (P_p - P_c)/h; in y the driving force carries
+// + g*ro_c (cpu.cpp:699-705).
+CAPAZ_HD double flux_x(double dm, double dp, double Xic, double roc, double Xim, double rom, double Xip, double rop,
+                       double h, double rh) {
+    return directed(-1. * dm, -1. * dp, Xic, roc, Xim, rom, Xip, rop, h, rh);
+}
+CAPAZ_HD double flux_y(const DevParams& p, double dm, double dp, double Xic, double roc, double Xim, double rom,
+                       double Xip, double rop) {
+    double gr = p.g * roc;
+    return directed(-1. * dm + gr, -1. * dp + gr, Xic, roc, Xim, rom, Xip, rop, p.hy, p.r_hy);
+}
+
+// ---- dense solve (gauss.cpp:6-104) -------------------------------------------------------
+
+// reverse_matrix on a row-major n x n matrix held in registers.  Same pivot
+// rule as the reference: pivot only if |a_kk| < 1e-8, then swap with every
+// later row whose (signed) entry exceeds 1e-8; if none, return -1 and leave
+// `a` as it was.  Row k is divided by the pivot through one correctly rounded
+// reciprocal and exact_div per entry.
+template <int N>
+CAPAZ_HD int reverse_matrix(double* a) {
+    constexpr int N1 = 2 * N;
+    const double e = 1E-8;
+    double m[N][N1];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+#pragma unroll
+        for (int j = 0; j < N; j++) m[i][j] = a[j + N * i];
+#pragma unroll
+        for (int j = N; j < N1; j++) m[i][j] = (j == N + i) ? 1. : 0.;
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        double f = m[k][k];
+        if (fabs(f) < e) {
+            bool found = false;
+#pragma unroll
+            for (int i = k + 1; i < N; i++) {
+                if (m[i][k] > e) {
+                    found = true;
+#pragma unroll
+                    for (int j = 0; j < N1; j++) {
+                        double t = m[k][j];
+                        m[k][j] = m[i][j];
+                        m[i][j] = t;
+                    }
+                    f = m[k][k];
+                }
+            }
+            if (!found) return -1;
+        }
+        double rf = 1. / f;
+#pragma unroll
+        for (int j = 0; j < N1; j++) m[k][j] = exact_div(m[k][j], f, rf);
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            if (i != k) {
+                double f1 = m[i][k];
+#pragma unroll
+                for (int j = 0; j < N1; j++) m[i][j] -= (m[k][j] * f1);
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = 0; j < N; j++) a[j + N * i] = m[i][j + N];
+    return 0;
+}
+
+// ---- Newton solves ------------------------------------------------------------------------
+
+// Two-phase fixed-iteration 2x2 Newton by Cramer's rule (two-phase.cpp:25-56).
+CAPAZ_HD void newton2(const DevParams& p, double& P, double& Sn, double roS_w, double roS_n) {
+    for (int w = 1; w <= p.iterations; w++) {
+        double P_k, PkS;
+        if (p.closure == 1) {
+            P_k = 0.;
+            PkS = 0.;
+        } else {
+            double S_e = exact_div(1 - Sn - p.S_wr, p.den2, p.r_den2);
+            P_k = p.P_d * pow(S_e, p.bc_epk);
+            double AAA = pow(S_e, p.bc_eaaa);
+            PkS = exact_div(AAA * p.P_d, p.bc_pks_den, p.r_bc_pks_den);
+        }
+        double F1 = p.ro0_w * (1. + p.beta_w * (P - p.P_atm)) * (1. - Sn) - roS_w;
+        double F2 = p.ro0_n * (1. + p.beta_n * (P + P_k - p.P_atm)) * Sn - roS_n;
+        double F1P = p.ro0_w * p.beta_w * (1 - Sn);
+        double F2P = p.ro0_n * p.beta_n * Sn;
+        double F1S = (-1) * p.ro0_w * (1 + p.beta_w * (P - p.P_atm));
+        double F2S = p.ro0_n * (1 + p.beta_n * (P + P_k - p.P_atm + (Sn * PkS)));
+        double det = F1P * F2S - F1S * F2P;
+        double rdet = 1 / det;
+        P = P - rdet * (F2S * F1 - F1S * F2);
+        Sn = Sn - rdet * (F1P * F2 - F2P * F1);
+    }
+}
+
+// Three-phase fixed-iteration 3x3 Newton (three-phase.cpp:286-339) with the
+// CPU path's zero capillary pressures folded; dF[4] = ro0_n*S_n keeps the
+// reference's Jacobian entry (three-phase.cpp:319).
+CAPAZ_HD void newton3(const DevParams& p, double& P, double& Sw, double& Sn, double roS_w, double roS_n,
+                      double roS_g) {
+    for (int w = 1; w <= p.iterations; w++) {
+        double Sg = 1. - Sw - Sn;
+        double rw = p.ro0_w * (1. + p.beta_w * (P - p.P_atm));  // dF[1]
+        double rn = p.ro0_n * (1. + p.beta_n * (P - p.P_atm));  // dF[5]
+        double gP = exact_div(p.ro0_g * P, p.P_atm, p.r_P_atm);
+        double F1 = rw * Sw - roS_w;
+        double F2 = rn * Sn - roS_n;
+        double F3 = gP * Sg - roS_g;
+        double dF[9];
+        dF[0] = p.c_wb * Sw;
+        dF[1] = rw;
+        dF[2] = 0;
+        dF[3] = p.c_nb * Sn;
+        dF[4] = p.ro0_n * Sn;
+        dF[5] = rn;
+        dF[6] = exact_div(p.ro0_g * Sg, p.P_atm, p.r_P_atm);
+        dF[7] = -gP;
+        dF[8] = -gP;
+        reverse_matrix<3>(dF);
+        P = P - (dF[0] * F1 + dF[1] * F2 + dF[2] * F3);
+        Sw = Sw - (dF[3] * F1 + dF[4] * F2 + dF[5] * F3);
+        Sn = Sn - (dF[6] * F1 + dF[7] * F2 + dF[8] * F3);
+    }
+}
+
+// ---- energy closures (energy.cpp:4-143) ---------------------------------------------------
+
+constexpr double kRoR = 2000.;
+constexpr double kLam0W = 0.553, kLam0N = 0.14, kLam0G = 0.0237, kLam0R = 1.;
+constexpr double kLamAW = 3E-3, kLamAN = 1E-3, kLamAG = 0.82;
+constexpr double kC0W = 4.194E3, kC0N = 1.7E3, kC0G = 1E3, kC0R = 0.8E3;
+constexpr double kCW = 1.15, kCW2 = 0.015, kCN = 3.4, kCG = 0.119, kCR = 0.75;
+
+CAPAZ_HD double c_w_T(double T) { return kC0W - kCW * (T - kT0) + kCW2 * (T - kT0) * (T - kT0); }
+CAPAZ_HD double c_n_T(double T) { return kC0N + kCN * (T - kT0); }
+CAPAZ_HD double c_g_T(double T) { return kC0G + kCG * (T - kT0); }
+CAPAZ_HD double c_r_T(double T) { return kC0R + kCR * (T - kT0); }
+
+CAPAZ_HD double lambda_eff(double m, double Sw, double Sn, double Sg, double T) {
+    double lw = kLam0W * (1 - kLamAW * (T - kT0));
+    double ln = kLam0N * (1 - kLamAN * (T - kT0));
+    double lg = kLam0G * pow(exact_div(T, kT0, 1. / kT0), kLamAG);
+    return m * (Sw * lw + Sn * ln + Sg * lg) + (1. - m) * kLam0R;
+}
+
+CAPAZ_HD double H_w_T(const DevParams& p, double T) {
+    return p.H0_w + (T - kT0) * (kC0W - (T - kT0) * (kCW * 0.5 + exact_div(kCW2 * (T - kT0), 3., 1. / 3.)));
+}
+CAPAZ_HD double H_n_T(const DevParams& p, double T) { return p.H0_n + (T - kT0) * (kC0N + half(kCN * (T - kT0))); }
+CAPAZ_HD double H_g_T(const DevParams& p, double T) { return p.H0_g + (T - kT0) * (kC0G + half(kCG * (T - kT0))); }
+CAPAZ_HD double H_r_T(const DevParams& p, double T) { return p.H0_r + (T - kT0) * (kC0R + half(kCR * (T - kT0))); }
+
+// mult_matrix_vector (gauss.cpp:94-104).
+template <int N>
+CAPAZ_HD void mult_matrix_vector(double* r, const double* m, const double* v) {
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        double s = 0;
+#pragma unroll
+        for (int j = 0; j < N; j++) s += m[j + N * i] * v[j];
+        r[i] = s;
+    }
+}
+
+// Energy-model 5x5 Newton in (S_w, S_n, S_g, P_w, T) (energy.cpp:370-463);
+// H_* are refreshed after every iteration as assign_H does.
+CAPAZ_HD void newton_energy(const DevParams& p, double& Sw, double& Sn, double& Sg, double& P, double& T, double& Hw,
+                            double& Hn, double& Hg, double& Hr, double m, double roS_w, double roS_n, double roS_g,
+                            double E_new) {
+    constexpr int n = 5;
+    for (int w = 1; w <= p.iterations; w++) {
+        double rw = ro_w_T(p, P, T), rn = ro_n_T(p, P, T), rg = ro_g_T(p, P, T);
+        double dwP = p.ro0_w * p.beta_w, dwT = (-1) * p.ro0_w * kAlfaW;
+        double dnP = p.ro0_n * p.beta_n, dnT = (-1) * p.ro0_n * kAlfaN;
+        double dgP = exact_div(p.ro0_g, p.P_atm, p.r_P_atm) * (kT0 / T);
+        double dgT = p.ro0_g * exact_div(P, p.P_atm, p.r_P_atm) * (-1) * (kT0 / T) / T;
+        double F[5], dF[25], corr[5];
+        F[0] = Sg + Sw + Sn - 1.;
+        F[1] = rw * Sw - roS_w;
+        F[2] = rn * Sn - roS_n;
+        F[3] = rg * Sg - roS_g;
+        F[4] = m * (Sw * (rw * Hw - P) + Sn * (rn * Hn - P) + Sg * (rg * Hg - P)) + (1. - m) * (kRoR * Hr - P) - E_new;
+        dF[0] = 1.; dF[1] = 1.; dF[2] = 1.; dF[3] = 0.; dF[4] = 0.;
+        dF[0 + n] = rw; dF[1 + n] = 0.; dF[2 + n] = 0.; dF[3 + n] = dwP * Sw; dF[4 + n] = dwT * Sw;
+        dF[0 + n * 2] = 0.; dF[1 + n * 2] = rn; dF[2 + n * 2] = 0.; dF[3 + n * 2] = dnP * Sn; dF[4 + n * 2] = dnT * Sn;
+        dF[0 + n * 3] = 0.; dF[1 + n * 3] = 0.; dF[2 + n * 3] = rg; dF[3 + n * 3] = dgP * Sg; dF[4 + n * 3] = dgT * Sg;
+        dF[0 + n * 4] = m * (rw * Hw - P);
+        dF[1 + n * 4] = m * (rn * Hn - P);
+        dF[2 + n * 4] = m * (rg * Hg - P);
+        dF[3 + n * 4] = m * (Sw * (dwP * Hw - 1.) + Sn * (dnP * Hn - 1.) + Sg * (dgP * Hg - 1.)) + (1. - m) * (-1);
+        dF[4 + n * 4] = m * (Sw * (dwT * Hw + rw * c_w_T(T)) + Sn * (dnT * Hn + rn * c_n_T(T)) +
+                             Sg * (dgT * Hg + rg * c_g_T(T))) +
+                        (1. - m) * kRoR * c_r_T(T);
+        reverse_matrix<5>(dF);
+        mult_matrix_vector<5>(corr, dF, F);
+        Sw = Sw - corr[0];
+        Sn = Sn - corr[1];
+        Sg = Sg - corr[2];
+        P = P - corr[3];
+        T = T - corr[4];
+        Hw = H_w_T(p, T);
+        Hn = H_n_T(p, T);
+        Hg = H_g_T(p, T);
+        Hr = H_r_T(p, T);
+    }
+}
+
+// ---- three-phase Border_P faces (three-phase.cpp:383-422) ---------------------------------
+
+// Sets (P_w, P_n, P_g) of a boundary cell in row j from the inward
+// neighbour's P_w.
+CAPAZ_HD void border_p3(const DevParams& p, int j, double P1, double& Pw, double& Pn, double& Pg) {
+    if ((j != 0) && (j != p.Ny - 1)) {
+        Pw = P1;
+        Pn = P1;
+        Pg = P1;
+    } else if (j == 0) {
+        Pw = exact_div(P1 - p.bp_w_num, p.bp_w_top_den, p.r_bp_w_top_den);
+        Pn = exact_div(P1 - p.bp_n_num, p.bp_n_top_den, p.r_bp_n_top_den);
+        Pg = exact_div(P1, p.bp_g_top_den, p.r_bp_g_top_den);
+    } else {
+        Pw = exact_div(P1 + p.bp_w_num, p.bp_w_bot_den, p.r_bp_w_bot_den);
+        Pn = exact_div(P1 + p.bp_n_num, p.bp_n_bot_den, p.r_bp_n_bot_den);
+        Pg = exact_div(P1, p.bp_g_bot_den, p.r_bp_g_bot_den);
+    }
+}
+
+// Two-phase Border_S window test at the top face (two-phase.cpp:175-181).
+CAPAZ_HD bool in_source_window(const DevParams& p, int i, int k) {
+    return (i >= p.Nx / 2 - p.source) && (i <= p.Nx / 2 + p.source) && (k >= p.Nz / 2 - p.source) &&
+           (k <= p.Nz / 2 + p.source);
+}
+
+}  // namespace capaz

@@ -1,0 +1,134 @@
+// HIP kernels of the backend for gfx950 and their launchers.  The per-cell
+// arithmetic lives in capaz_bodies.h / capaz_cell.h; the kernels here map
+// threads onto cells (x fastest, so a wavefront reads 64 consecutive doubles
+// of each field) and are launched on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include "capaz_bodies.h"
+#include "capaz_launch.h"
+
+namespace capaz {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__global__ void __launch_bounds__(kBlock) k_S(DevParams p, DevFields F) {
+    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l < p.n) body_S(p, F, l);
+}
+__global__ void __launch_bounds__(kBlock) k_P_Xi_ro(DevParams p, DevFields F) {
+    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l < p.n) body_P_Xi_ro(p, F, l);
+}
+__global__ void __launch_bounds__(kBlock) k_u(DevParams p, DevFields F) {
+    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l < p.n) body_u(p, F, l);
+}
+__global__ void __launch_bounds__(kBlock) k_roS(DevParams p, DevFields F) {
+    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l < p.n) body_roS(p, F, l);
+}
+__global__ void __launch_bounds__(kBlock) k_HE(DevParams p, DevFields F) {
+    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l < p.n) body_HE(p, F, l);
+}
+__global__ void __launch_bounds__(kBlock) k_E(DevParams p, DevFields F) {
+    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l < p.n) body_E(p, F, l);
+}
+__global__ void __launch_bounds__(kBlock) k_newton(DevParams p, DevFields F) {
+    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l < p.n) body_newton(p, F, l);
+}
+__global__ void __launch_bounds__(kBlock) k_boundary(DevParams p, DevFields F) {
+    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l < p.n) body_boundary(p, F, l);
+}
+__global__ void __launch_bounds__(kBlock) k_fused_step(DevParams p, StepIO io) {
+    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l < p.n) body_fused_step(p, io, l);
+}
+
+// Copies the interior cells of src into dst (the three-phase P_n/P_g whose
+// interior values belong to the previous state after a fused step).
+__global__ void __launch_bounds__(kBlock) k_copy_interior(DevParams p, const double* src, double* dst) {
+    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l >= p.n) return;
+    int i, j, k;
+    decompose(p, l, i, j, k);
+    if (interior3(p, i, j, k)) dst[l] = src[l];
+}
+
+// Porosity check of the fused path: sets *flag if an interior cell has m == 0
+// (assign_roS_nr skips such cells, cpu.cpp:625-626).
+__global__ void __launch_bounds__(kBlock) k_m_zero(DevParams p, const double* m, int* flag) {
+    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l >= p.n) return;
+    int i, j, k;
+    decompose(p, l, i, j, k);
+    if (cell_inner(p, i, j, k) && !m[l]) atomicOr(flag, 1);
+}
+
+// Face planes (cpu.cpp:769-888): plane index `pos` normal to `axis`; the
+// buffer is indexed (a + na*b) with (a, b) = (j, k) for x, (i, k) for y,
+// (i, j) for z.
+__global__ void __launch_bounds__(kBlock) k_face(DevParams p, double* field, double* buf, int axis, int pos,
+                                                 int save) {
+    int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    int na = axis == 0 ? p.Ny : p.Nx;
+    int nb = axis == 2 ? p.Ny : p.Nz;
+    if (t >= (int64_t)na * nb) return;
+    int a = (int)(t % na), b = (int)(t / na);
+    int i, j, k;
+    if (axis == 0) { i = pos; j = a; k = b; }
+    else if (axis == 1) { i = a; j = pos; k = b; }
+    else { i = a; j = b; k = pos; }
+    int64_t l = (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz;
+    if (save)
+        field[l] = buf[t];
+    else
+        buf[t] = field[l];
+}
+
+inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
+
+}  // namespace
+
+hipError_t launch_entry(Entry e, const DevParams& p, const DevFields& F, hipStream_t s) {
+    dim3 g = grid_for(p.n), b(kBlock);
+    switch (e) {
+        case Entry::S: hipLaunchKernelGGL(k_S, g, b, 0, s, p, F); break;
+        case Entry::P_Xi_ro: hipLaunchKernelGGL(k_P_Xi_ro, g, b, 0, s, p, F); break;
+        case Entry::U: hipLaunchKernelGGL(k_u, g, b, 0, s, p, F); break;
+        case Entry::RoS: hipLaunchKernelGGL(k_roS, g, b, 0, s, p, F); break;
+        case Entry::HE: hipLaunchKernelGGL(k_HE, g, b, 0, s, p, F); break;
+        case Entry::E: hipLaunchKernelGGL(k_E, g, b, 0, s, p, F); break;
+        case Entry::Newton: hipLaunchKernelGGL(k_newton, g, b, 0, s, p, F); break;
+        case Entry::Boundary: hipLaunchKernelGGL(k_boundary, g, b, 0, s, p, F); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fused_step(const DevParams& p, const StepIO& io, hipStream_t s) {
+    hipLaunchKernelGGL(k_fused_step, grid_for(p.n), dim3(kBlock), 0, s, p, io);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_interior(const DevParams& p, const double* src, double* dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_interior, grid_for(p.n), dim3(kBlock), 0, s, p, src, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_m_zero(const DevParams& p, const double* m, int* flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_m_zero, grid_for(p.n), dim3(kBlock), 0, s, p, m, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_face(const DevParams& p, double* field, double* buf, int axis, int pos, int save, hipStream_t s) {
+    int64_t na = axis == 0 ? p.Ny : p.Nx, nb = axis == 2 ? p.Ny : p.Nz;
+    hipLaunchKernelGGL(k_face, grid_for(na * nb), dim3(kBlock), 0, s, p, field, buf, axis, pos, save);
+    return hipGetLastError();
+}
+
+}  // namespace capaz

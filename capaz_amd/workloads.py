@@ -1,0 +1,76 @@
+"""Seeded synthetic inputs of the benchmark configurations.
+
+Each builder returns ``(keys, fields)``: a deck of defines.ini keys
+(main.cpp:1327-1626) plus the backend's MODEL / ENERGY / CLOSURE / COREY_N
+keys, and the initial primary fields as (Nz, Ny, Nx) float64 arrays, x
+fastest.  Random fields come from numpy's PCG64 with seed 1000 + config
+index.  "Top" is j = 0 (gravity along +j, cpu.cpp:392).
+
+Parameter substitutions (the decks as specified are unstable for this
+explicit scheme; the stable values used are):
+  * three-phase (config 1 and the 256^3 headline): dt = 1e-5 s instead of
+    1e-4 s.  At h = 0.01 m and beta = 4.5e-10 1/Pa the scheme diverges within
+    20 steps at dt = 1e-4 (24^3, oracle and reference alike); the threshold
+    measured at 24^3 is ~6e-5 s and lognormal K tails at 256^3 tighten it.
+  * two-phase (config 0): beta_w = beta_n = 1e-6 1/Pa instead of 4.5e-10.
+    Without gas the only compressibility is beta, and the pressure-diffusion
+    limit h^2*phi*beta*mu/(6*K*k) is ~2.5e-7 s at 4.5e-10, 400x below
+    dt = 1e-4; at 1e-6 it is ~5.6e-4 s.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+P_ATM = 1.0e5
+K_MEAN = 6.64e-11
+
+
+def _porosity_noise(rng, shape):
+    return np.clip(0.4 * (1.0 + 0.05 * rng.standard_normal(shape)), 0.3, 0.5)
+
+
+def config0(n: int = 40):
+    """Two-phase water/NAPL, n^3, Corey-2 (S_wr 0.1, S_nr 0.05), scalar K,
+    S_w0 = 0.15, top face injects S_w = 0.9 at P = 1.2e5 Pa.
+
+    The reference's two-phase boundary sets the top pressure to P_atm and the
+    bottom by a hydrostatic increment (two-phase.cpp:200-211), so the top
+    pressure is carried by P_ATM = 1.2e5 and the bottom is the reference's
+    hydrostatic rule, not a fixed 1e5 Pa."""
+    rng = np.random.default_rng(1000)
+    shape = (n, n, n)
+    keys = dict(MODEL=2, CLOSURE="corey", COREY_N=2, NX=n, NY=n, NZ=n, HX=0.01, HY=0.01, HZ=0.01, DT=1e-4,
+                P_ATM=1.2e5, G_CONST=9.807, MU_W=1e-3, MU_N=5e-3, RO_W=1000.0, RO_N=850.0,
+                BETA_W=1e-6, BETA_N=1e-6, S_WR_0=0.1, S_NR_0=0.05, SOURCE=n, S_N_GR=0.1,
+                ITERATIONS=4, K_0=K_MEAN)
+    P = np.full(shape, keys["P_ATM"])
+    fields = dict(P_w=P, S_n=np.full(shape, 0.85), m=_porosity_noise(rng, shape), K=np.full(shape, K_MEAN))
+    return keys, fields
+
+
+def config1(n: int = 128, seed: int = 1001):
+    """Three-phase water/NAPL/gas on an n^3 grid (n = 128 in config 1; the
+    headline measures the same physics at n = 256): 4 Newton iterations,
+    ideal-gas density with ro0_g = 1.2, mu_g = 1.8e-5, Corey-2 relperms
+    (S_wr 0.1, S_nr 0.05, S_gr 0), S0 = (0.2, 0.7, 0.1), the top face injects
+    (0.6, 0.1, 0.3), porosity 0.4(1 + 0.05 N(0,1)) clipped to [0.3, 0.5],
+    K lognormal with geometric mean 6.64e-11 m^2 and ln-sigma 0.5 (white
+    noise per cell).  h = 0.01 m, beta = 4.5e-10 1/Pa, mu_w 1e-3, mu_n 5e-3,
+    ro0 1000 / 850 as config 0; P = P_atm everywhere initially (the
+    reference's three-phase data_initialization, three-phase.cpp:470-472)."""
+    rng = np.random.default_rng(seed)
+    shape = (n, n, n)
+    keys = dict(MODEL=3, CLOSURE="corey", COREY_N=2, NX=n, NY=n, NZ=n, HX=0.01, HY=0.01, HZ=0.01, DT=1e-5,
+                P_ATM=P_ATM, G_CONST=9.807, MU_W=1e-3, MU_N=5e-3, MU_G=1.8e-5, RO_W=1000.0, RO_N=850.0,
+                RO_G=1.2, BETA_W=4.5e-10, BETA_N=4.5e-10, S_WR_0=0.1, S_NR_0=0.05, S_GR_0=0.0,
+                SOURCE=1, S_W_GR=0.6, S_N_GR=0.1, ITERATIONS=4, K_0=K_MEAN)
+    m = _porosity_noise(rng, shape)
+    K = np.exp(np.log(K_MEAN) + 0.5 * rng.standard_normal(shape))
+    P = np.full(shape, P_ATM)
+    fields = dict(P_w=P, P_n=P.copy(), P_g=P.copy(), S_w=np.full(shape, 0.2), S_n=np.full(shape, 0.7), m=m, K=K)
+    return keys, fields
+
+
+def headline(n: int = 256):
+    """The benchmark workload: config 1 physics at 256^3 (seed 1001)."""
+    return config1(n, seed=1001)

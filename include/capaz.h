@@ -1,0 +1,176 @@
+/* capaz_amd: MI355X (gfx950) backend for capaz's explicit 3-D two- and
+ * three-phase filtration step, optionally with the energy equation.
+ *
+ * This is the drop-in boundary: a C ABI (plain pointers and sizes, no C++ or
+ * torch types) that replaces the device half of the reference's compute
+ * backend, gpu.cu, behind the entry points its driver calls through
+ * defines.h.  Each entry point below names the reference function it stands
+ * in for.  A C++ adapter with the exact defines.h signatures
+ * (capaz_amd/adapter/capaz_gpu_backend.cpp) forwards to these functions, so a
+ * driver with main.cpp's structure links against this library the way it
+ * linked against gpu.o.
+ *
+ * Storage: one FP64 array per field on the device, x fastest,
+ * local = i + j*Nx + k*Nx*Ny (cpu.cpp:145), j = 0 the top face.
+ *
+ * Errors: every function returns CAPAZ_OK (0) or a negative capaz_status;
+ * capaz_last_error() gives the message of the last failure on a context (or,
+ * for a NULL context, of the last failed capaz_create on this thread).  The
+ * reference's functions return void and exit(1) on error (shared_test.cpp:23-33);
+ * this ABI reports instead.
+ *
+ * Threading: a context is used by one host thread at a time; different
+ * contexts are independent.  All work of a context is ordered on its own HIP
+ * stream; functions that return data to the host synchronise that stream. */
+#ifndef CAPAZ_H
+#define CAPAZ_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CAPAZ_ABI_VERSION 1
+
+typedef struct capaz_ctx capaz_ctx;
+
+typedef enum capaz_status {
+    CAPAZ_OK = 0,
+    CAPAZ_ERR_INVALID = -1, /* bad argument or unsupported configuration */
+    CAPAZ_ERR_DEVICE = -2,  /* a HIP call failed */
+    CAPAZ_ERR_NOMEM = -3,   /* device allocation failed */
+    CAPAZ_ERR_STATE = -4    /* call not valid in the context's current state */
+} capaz_status;
+
+typedef enum capaz_model {
+    CAPAZ_TWO_PHASE = 2,  /* water / NAPL   (Two-phase/two-phase.cpp) */
+    CAPAZ_THREE_PHASE = 3 /* water / NAPL / gas (Three-phase/three-phase.cpp) */
+} capaz_model;
+
+typedef enum capaz_closure {
+    /* The reference's CPU closures: Brooks-Corey k_r and P_c for two phases
+     * (two-phase.cpp:4-23); van Genuchten-type k_r and P_c = 0 for three
+     * phases (three-phase.cpp:11-186). */
+    CAPAZ_CLOSURE_REFERENCE = 0,
+    /* Corey k_r = S_e^n of the effective saturations clipped to [0, 1], no
+     * capillary pressure.  No reference counterpart. */
+    CAPAZ_CLOSURE_COREY = 1
+} capaz_closure;
+
+/* Field identifiers, in the member order of ptr_Arrays (defines.h:65-76). */
+typedef enum capaz_field_id {
+    CAPAZ_S_n = 0, CAPAZ_S_w, CAPAZ_P_w, CAPAZ_P_n, CAPAZ_ro_w, CAPAZ_ro_n,
+    CAPAZ_ux_w, CAPAZ_uy_w, CAPAZ_uz_w, CAPAZ_ux_n, CAPAZ_uy_n, CAPAZ_uz_n,
+    CAPAZ_Xi_w, CAPAZ_Xi_n, CAPAZ_roS_w, CAPAZ_roS_w_old, CAPAZ_roS_n, CAPAZ_roS_n_old,
+    CAPAZ_m, CAPAZ_K,
+    CAPAZ_S_g, CAPAZ_P_g, CAPAZ_ro_g, CAPAZ_ux_g, CAPAZ_uy_g, CAPAZ_uz_g, CAPAZ_Xi_g,
+    CAPAZ_roS_g, CAPAZ_roS_g_old,
+    CAPAZ_T, CAPAZ_H_w, CAPAZ_H_n, CAPAZ_H_g, CAPAZ_H_r, CAPAZ_E, CAPAZ_E_new,
+    CAPAZ_NUM_FIELDS
+} capaz_field_id;
+
+/* Parameters of one domain: the subset of the reference's `consts` block
+ * (defines.h:79-113) that the step reads, media 0 only, plus the closure
+ * selector.  Field names follow the defines.ini keys in brackets. */
+typedef struct capaz_params {
+    unsigned struct_size;  /* sizeof(capaz_params), for ABI checks */
+    int model;             /* capaz_model */
+    int energy;            /* 1: energy equation (three-phase only, as energy.cpp) */
+    int closure;           /* capaz_closure */
+    int Nx, Ny, Nz;        /* [NX] [NY] [NZ] */
+    int source;            /* [SOURCE] top-face injection half-width (>0: inject) */
+    int newton_iterations; /* [ITERATIONS] */
+    int device;            /* HIP device ordinal */
+    double hx, hy, hz, dt; /* [HX] [HY] [HZ] [DT] */
+    double P_atm, g_const; /* [P_ATM] [G_CONST] */
+    double mu_w, mu_n, mu_g;    /* [MU_W] [MU_N] [MU_G] */
+    double ro0_w, ro0_n, ro0_g; /* [RO_W] [RO_N] [RO_G] */
+    double beta_w, beta_n;      /* [BETA_W] [BETA_N] */
+    double lambda;              /* [LAMBDA_0] */
+    double S_wr, S_nr, S_gr;    /* [S_WR_0] [S_NR_0] [S_GR_0] */
+    double P_d;                 /* [P_D_0] two-phase entry pressure */
+    double S_w_gr, S_n_gr;      /* [S_W_GR] [S_N_GR] injected saturations */
+    double corey_n;             /* Corey exponent (CAPAZ_CLOSURE_COREY) */
+} capaz_params;
+
+/* Fills *p with zeros, struct_size and defaults (newton_iterations = 1,
+ * corey_n = 2). */
+void capaz_params_init(capaz_params* p);
+
+/* Library ABI version (CAPAZ_ABI_VERSION of the build). */
+int capaz_abi_version(void);
+
+/* device_initialization + device_memory_allocation (gpu.cu:653-852;
+ * defines.h:121,138).  Allocates every field of the model on the device,
+ * zero-filled, and a stream.  Grids must have Ny >= 3 and Nx, Nz each 1 or
+ * >= 3 (Nx >= 3 for two phases), so that every boundary cell's inward
+ * neighbour (cpu.cpp:107-141) is an interior cell. */
+int capaz_create(const capaz_params* p, capaz_ctx** out);
+
+/* device_memory_free + device_finalization (defines.h:124,139). */
+int capaz_destroy(capaz_ctx* ctx);
+
+const char* capaz_last_error(const capaz_ctx* ctx);
+
+/* Number of cells, Nx*Ny*Nz. */
+size_t capaz_cells(const capaz_ctx* ctx);
+
+/* 1 if `field` exists for the context's model. */
+int capaz_has_field(const capaz_ctx* ctx, int field);
+
+/* load_data_to_device (gpu.cu:660-667; defines.h:135): copies n = Nx*Ny*Nz
+ * doubles from host memory into a field. */
+int capaz_set_field(capaz_ctx* ctx, int field, const double* host, size_t n);
+
+/* load_data_to_host (gpu.cu:653-657; defines.h:134): copies a field to host
+ * memory.  After capaz_step every field reads as it would after the same
+ * number of reference time steps. */
+int capaz_get_field(capaz_ctx* ctx, int field, double* host, size_t n);
+
+/* Device address of a field's current array (valid until the next call on
+ * the context), for callers that share device memory, e.g. torch tensors. */
+int capaz_field_device_ptr(capaz_ctx* ctx, int field, double** dptr);
+
+/* Device-to-device copy of n doubles into a field (stream-ordered). */
+int capaz_set_field_device(capaz_ctx* ctx, int field, const double* dsrc, size_t n);
+
+/* Per-entry-point kernels, each with the reference CPU path's semantics
+ * (cpu.cpp, the model .cpp, gauss.cpp, energy.cpp), launched on the
+ * context's stream without a host synchronisation. */
+int capaz_S(capaz_ctx* ctx);                  /* S_calculation        cpu.cpp:28-36 */
+int capaz_P_Xi_ro(capaz_ctx* ctx);            /* ro_P_Xi_calculation  cpu.cpp:3-15 */
+int capaz_u(capaz_ctx* ctx);                  /* u_calculation        cpu.cpp:17-26 */
+int capaz_roS(capaz_ctx* ctx, double t);      /* roS_calculation      cpu.cpp:52-65 */
+int capaz_energy_HE(capaz_ctx* ctx);          /* H_E_current_calculation cpu.cpp:39-49 */
+int capaz_energy_E(capaz_ctx* ctx);           /* E_calculation        cpu.cpp:68-77 */
+int capaz_newton(capaz_ctx* ctx);             /* P_S_calculation      cpu.cpp:80-89 */
+int capaz_boundary(capaz_ctx* ctx);           /* boundary_conditions  cpu.cpp:91-104 */
+
+/* time_step_function (main.cpp:93-108) as the sequence of the entry-point
+ * kernels above, at time t. */
+int capaz_step_sequence(capaz_ctx* ctx, double t);
+
+/* nsteps time steps, step numbers first_step .. first_step+nsteps-1
+ * (t = step*dt as in main.cpp:49), with the fused kernels: the same results
+ * as capaz_step_sequence, bit for bit, in fewer passes over memory. */
+int capaz_step(capaz_ctx* ctx, long first_step, long nsteps);
+
+/* Waits for all work queued on the context. */
+int capaz_sync(capaz_ctx* ctx);
+
+/* The context's hipStream_t, as void*. */
+int capaz_stream(capaz_ctx* ctx, void** stream);
+
+/* Face exchange buffers of the multi-domain reference (gpu.cu:854-1128;
+ * defines.h:196-207): copies the second (side 0) or second-to-last (side 1)
+ * plane normal to `axis` (0 x, 1 y, 2 z) of a field into `dbuf` (load), or
+ * `dbuf` into the first / last plane (save).  `dbuf` is device memory of the
+ * plane's size; the plane is laid out as the reference's HostBuffer. */
+int capaz_load_face(capaz_ctx* ctx, int field, int axis, int side, double* dbuf);
+int capaz_save_face(capaz_ctx* ctx, int field, int axis, int side, const double* dbuf);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

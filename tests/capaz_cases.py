@@ -1,0 +1,105 @@
+"""Seeded parameter sets and initial states shared by the tests.
+
+Parameter sets are written as defines.ini keys (main.cpp:1327-1626) plus the
+backend's MODEL / ENERGY / CLOSURE / COREY_N keys.  ``REF_2PH`` and
+``REF_3PH`` hold the values of the reference's own input decks
+(Two-phase/defines.ini, Three-phase/defines.ini); the grid keys are set per
+case.  States are drawn from numpy's PCG64 with an explicit seed so that the
+checker and the device see identical arrays.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+# Two-phase/defines.ini (values only).
+REF_2PH = dict(
+    MODEL=2, HX=5e-2, HY=5e-2, HZ=5e-2, DT=1e-3, TAU=1e-7, SOURCE=30, L=10e-5, C_N=1000, C_W=1500,
+    BETA_W=4.4e-7, BETA_N=1e-6, K_0=6.64e-11, K_1=7.15e-12, MU_W=0.001, MU_N=0.0009, RO_W=1000,
+    RO_N=1460, LAMBDA_0=2.7, LAMBDA_1=2.0, S_WR_0=0.09, S_WR_1=0.12, S_N_GR=0.2, M_0=0.4, M_1=0.39,
+    P_D_0=755, P_D_1=2060, P_ATM=100000, G_CONST=9.807, ITERATIONS=10,
+)
+
+# Three-phase/defines.ini (values only).
+REF_3PH = dict(
+    MODEL=3, HX=5e-2, HY=5e-2, HZ=5e-2, DT=1e-3, TAU=2e-3, S_W_GR=0.4, S_N_GR=0.3, S_W_INIT=0.4,
+    S_N_INIT=0.3, SOURCE=0, L=1e-7, C_N=1000, C_W=1500, C_G=330, BETA_W=4.4e-7, BETA_N=1.0e-6,
+    BETA_G=9.2e-6, MU_W=1e-3, MU_N=1e-3, MU_G=1.84e-5, RO_W=1000, RO_N=850, RO_G=1.4, P_ATM=1.0e5,
+    G_CONST=9.807, ITERATIONS=7, LAMBDA_0=3.25, LAMBDA_1=3.25, P_D_NW_0=3125.0, P_D_NW_1=3125.0,
+    P_D_GN_0=1042.0, P_D_GN_1=1042.0, K_0=6.64e-11, K_1=7.15e-12, M_0=0.4, M_1=0.39, S_WR_0=0.0,
+    S_WR_1=0.0, S_NR_0=0.0, S_NR_1=0.0, S_GR_0=0.0, S_GR_1=0.0, Q=0.15e-1,
+)
+
+# Keys of the backend only (not defines.ini keys of the reference).
+BACKEND_KEYS = ("MODEL", "ENERGY", "CLOSURE", "COREY_N")
+
+
+def ref_keys(keys: dict) -> dict:
+    """The subset of ``keys`` that the reference's consts block understands."""
+    return {k: v for k, v in keys.items() if k not in BACKEND_KEYS}
+
+
+def case(model: str, nx: int, ny: int, nz: int, **over) -> dict:
+    """Reference deck for ``model`` ("2ph", "3ph", "3phe") on an nx*ny*nz grid."""
+    if model == "2ph":
+        k = dict(REF_2PH)
+    else:
+        k = dict(REF_3PH)
+        if model == "3phe":
+            k["ENERGY"] = 1
+            k["ITERATIONS"] = 3
+    k.update(NX=nx, NY=ny, NZ=nz)
+    k.update(over)
+    return k
+
+
+def random_state(keys: dict, seed: int, hetero_m: bool = True) -> dict[str, np.ndarray]:
+    """A smooth-plus-noise initial state of the primary fields.
+
+    P_w is hydrostatic around P_ATM with +-50 Pa noise; saturations are
+    interior to (0, 1); porosity is 0.4*(1 + 0.05 N(0,1)) clipped to
+    [0.3, 0.5]; K is the deck's scalar K_0 in every cell (the reference reads
+    def.K[0]).  For three phases, P_n and P_g get the boundary-persistent
+    values P_w + noise (the reference keeps them from the previous
+    Border_P)."""
+    rng = np.random.default_rng(seed)
+    nz, ny, nx = int(keys["NZ"]), int(keys["NY"]), int(keys["NX"])
+    shape = (nz, ny, nx)
+    model = int(keys.get("MODEL", 3))
+    j = np.arange(ny).reshape(1, ny, 1)
+    P = keys["P_ATM"] + 1000.0 * keys["G_CONST"] * keys["HY"] * j + rng.uniform(-50, 50, shape)
+    st = {"P_w": P}
+    if hetero_m:
+        st["m"] = np.clip(0.4 * (1 + 0.05 * rng.standard_normal(shape)), 0.3, 0.5)
+    else:
+        st["m"] = np.full(shape, keys.get("M_0", 0.4))
+    st["K"] = np.full(shape, keys["K_0"])
+    if model == 2:
+        st["S_n"] = rng.uniform(0.05, 0.4, shape)
+    else:
+        st["S_w"] = rng.uniform(0.25, 0.5, shape)
+        st["S_n"] = rng.uniform(0.15, 0.35, shape)
+        st["P_n"] = P + rng.uniform(-5, 5, shape)
+        st["P_g"] = P + rng.uniform(-5, 5, shape)
+        if int(keys.get("ENERGY", 0)):
+            st["T"] = 285.0 + rng.uniform(-3, 3, shape)
+    return st
+
+
+def max_rel(a: np.ndarray, b: np.ndarray) -> float:
+    """max |a-b| / max(|a|,|b|) with 0/0 = 0 and NaN == NaN."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    both_nan = np.isnan(a) & np.isnan(b)
+    d = np.abs(a - b)
+    s = np.maximum(np.abs(a), np.abs(b))
+    r = np.where(d == 0, 0.0, d / np.where(s == 0, 1.0, s))
+    r = np.where(both_nan, 0.0, r)
+    r = np.where(np.isnan(r), np.inf, r)
+    return float(r.max()) if r.size else 0.0
+
+
+def bit_equal(a: np.ndarray, b: np.ndarray) -> bool:
+    """Equal as numbers (+0 == -0), with NaN == NaN."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))

@@ -1,0 +1,60 @@
+// Host build of the backend's per-cell kernel bodies (test infrastructure).
+//
+// The HIP kernels are thin loops over the bodies in capaz_bodies.h; this file
+// runs the same bodies in plain loops on the CPU, so the tests can compare
+// the kernels' arithmetic with the oracle bit for bit without a GPU.  The
+// only difference from the device build is the math library behind pow/exp
+// (glibc here, the ROCm device library on the GPU).
+#include <cstdint>
+
+#include "capaz_bodies.h"
+#include "capaz_derive.h"
+
+extern "C" {
+
+const char* capaz_emul_validate(const capaz_params* u) {
+    static thread_local std::string why;
+    why = capaz::validate(*u);
+    return why.c_str();
+}
+
+// entry: 0 S, 1 P_Xi_ro, 2 u, 3 roS, 4 HE, 5 E, 6 Newton, 7 boundary.
+int capaz_emul_entry(const capaz_params* u, double** fields, int entry) {
+    if (!capaz::validate(*u).empty()) return -1;
+    capaz::DevParams p = capaz::derive(*u);
+    capaz::DevFields F;
+    for (int f = 0; f < capaz::F_COUNT; ++f) F.f[f] = fields[f];
+    for (int64_t l = 0; l < p.n; ++l) {
+        switch (entry) {
+            case 0: capaz::body_S(p, F, l); break;
+            case 1: capaz::body_P_Xi_ro(p, F, l); break;
+            case 2: capaz::body_u(p, F, l); break;
+            case 3: capaz::body_roS(p, F, l); break;
+            case 4: capaz::body_HE(p, F, l); break;
+            case 5: capaz::body_E(p, F, l); break;
+            case 6: capaz::body_newton(p, F, l); break;
+            case 7: capaz::body_boundary(p, F, l); break;
+            default: return -1;
+        }
+    }
+    return 0;
+}
+
+// One fused step: cur = {Pw, Sw, Sn, Pn, Pg, m, K}, next = {Pw, Sw, Sn, Pn, Pg}.
+int capaz_emul_fused_step(const capaz_params* u, double** cur, double** next) {
+    if (!capaz::validate(*u).empty()) return -1;
+    capaz::DevParams p = capaz::derive(*u);
+    capaz::StepIO io;
+    io.Pw = cur[0]; io.Sw = cur[1]; io.Sn = cur[2]; io.Pn = cur[3]; io.Pg = cur[4]; io.m = cur[5]; io.K = cur[6];
+    io.Pw1 = next[0]; io.Sw1 = next[1]; io.Sn1 = next[2]; io.Pn1 = next[3]; io.Pg1 = next[4];
+    for (int64_t l = 0; l < p.n; ++l) capaz::body_fused_step(p, io, l);
+    return 0;
+}
+
+int capaz_emul_reverse_matrix(double* a, int n) {
+    if (n == 3) return capaz::reverse_matrix<3>(a);
+    if (n == 5) return capaz::reverse_matrix<5>(a);
+    return -2;
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+"""ctypes access to the host build of the kernel bodies (test infrastructure).
+
+``EmulDomain`` runs the per-cell bodies of the HIP kernels on numpy arrays on
+the CPU, with the same interface as ``oracle.OracleDomain``, so the kernels'
+arithmetic can be compared with the oracle bit for bit without a GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from capaz_amd.capaz import FIELDS, Params, params_from_keys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libcapaz_emul.so")
+
+ENTRY = {"S_calculation": 0, "ro_P_Xi_calculation": 1, "u_calculation": 2, "roS_calculation": 3,
+         "H_E_current_calculation": 4, "E_calculation": 5, "P_S_calculation": 6, "boundary_conditions": 7}
+
+_lib = None
+
+
+def available() -> bool:
+    return os.path.exists(LIB)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(LIB)
+        PP = ctypes.POINTER(ctypes.POINTER(ctypes.c_double))
+        L.capaz_emul_entry.argtypes = [ctypes.POINTER(Params), PP, ctypes.c_int]
+        L.capaz_emul_fused_step.argtypes = [ctypes.POINTER(Params), PP, PP]
+        L.capaz_emul_reverse_matrix.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.capaz_emul_validate.argtypes = [ctypes.POINTER(Params)]
+        L.capaz_emul_validate.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def validate(keys: dict) -> str:
+    p = params_from_keys(keys)
+    return lib().capaz_emul_validate(ctypes.byref(p)).decode()
+
+
+def reverse_matrix(a: np.ndarray) -> tuple[int, np.ndarray]:
+    buf = np.ascontiguousarray(a, dtype=np.float64).copy()
+    rc = lib().capaz_emul_reverse_matrix(buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a.shape[0])
+    return rc, buf
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if a is not None else None
+
+
+class EmulDomain:
+    def __init__(self, keys: dict, fields: dict | None = None):
+        self.keys = dict(keys)
+        self.p = params_from_keys(keys)
+        why = lib().capaz_emul_validate(ctypes.byref(self.p)).decode()
+        if why:
+            raise ValueError(why)
+        self.shape = (self.p.Nz, self.p.Ny, self.p.Nx)
+        model, energy = self.p.model, self.p.energy
+        self.names = [f for i, f in enumerate(FIELDS)
+                      if i < 20 or (20 <= i < 29 and model == 3) or (i >= 29 and energy)]
+        self.arrays = {n: np.zeros(self.shape) for n in self.names}
+        if fields:
+            for n, v in fields.items():
+                if n in self.arrays:
+                    self.arrays[n][...] = v
+
+    def field(self, name):
+        return self.arrays[name]
+
+    def fields(self):
+        return {n: a.copy() for n, a in self.arrays.items()}
+
+    def _table(self):
+        arr = (ctypes.POINTER(ctypes.c_double) * len(FIELDS))()
+        for i, f in enumerate(FIELDS):
+            arr[i] = _ptr(self.arrays.get(f))
+        return arr
+
+    def call(self, entry: str, t: float = 0.0) -> None:
+        if entry == "time_step_function":
+            for e in ("S_calculation", "ro_P_Xi_calculation", "u_calculation", "roS_calculation"):
+                self.call(e, t)
+            if self.p.energy:
+                self.call("H_E_current_calculation")
+                self.call("E_calculation")
+            self.call("P_S_calculation")
+            self.call("boundary_conditions")
+            return
+        rc = lib().capaz_emul_entry(ctypes.byref(self.p), self._table(), ENTRY[entry])
+        if rc:
+            raise RuntimeError(f"emul entry {entry} failed")
+
+    def run(self, nsteps: int, first_step: int = 1) -> None:
+        for s in range(first_step, first_step + nsteps):
+            self.call("time_step_function", s * self.p.dt)
+
+    def fused_step(self) -> None:
+        """One fused-kernel step on the primary state (P_w, S_w, S_n, P_n, P_g)."""
+        names = ("P_w", "S_w", "S_n", "P_n", "P_g")
+        cur = [self.arrays.get(n) for n in names] + [self.arrays["m"], self.arrays["K"]]
+        nxt = [a.copy() if a is not None else None for a in cur[:5]]
+        C = (ctypes.POINTER(ctypes.c_double) * 7)(*[_ptr(a) for a in cur])
+        N = (ctypes.POINTER(ctypes.c_double) * 5)(*[_ptr(a) for a in nxt])
+        if lib().capaz_emul_fused_step(ctypes.byref(self.p), C, N):
+            raise RuntimeError("emul fused step failed")
+        # Primary state only: P_w, S_n (+ S_w, P_n, P_g for three phases; of
+        # P_n/P_g the kernel writes the boundary cells, the interior keeps the
+        # copied previous values).
+        primary = names if self.p.model == 3 else ("P_w", "S_n")
+        for n, a in zip(names, nxt):
+            if n in primary:
+                self.arrays[n][...] = a
