@@ -62,9 +62,7 @@ def main() -> int:
               "(the multi-GPU slab decomposition is a design, not a build)", file=sys.stderr)
         return 2
 
-    import torch
-
-    from capaz_amd import capaz, workloads
+    from capaz_amd import capaz, hip, workloads
 
     keys, st = workloads.headline(args.n)
     cells = args.n ** 3
@@ -74,18 +72,17 @@ def main() -> int:
     dom.run(args.warmup, first_step=1)
     dom.sync()
 
-    stream = torch.cuda.ExternalStream(dom.stream())
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
+    # HIP events on the context's stream, where the fused-step kernels run
+    stream = dom.stream()
+    ev0, ev1 = hip.Event(), hip.Event()
+    dom.sync()
     t0 = time.perf_counter()
     ev0.record(stream)
     dom.run(args.steps, first_step=args.warmup + 1)
     ev1.record(stream)
     dom.sync()
-    torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    gpu_ms = ev0.elapsed_time(ev1)
+    gpu_ms = ev0.elapsed_ms(ev1)
     print(f"bench: {args.steps} steps in {wall:.3f} s (events {gpu_ms:.1f} ms)", file=sys.stderr, flush=True)
 
     launch_s = gpu_ms / 1e3 / args.steps  # one fused-step kernel per step

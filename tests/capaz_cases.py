@@ -98,6 +98,23 @@ def max_rel(a: np.ndarray, b: np.ndarray) -> float:
     return float(r.max()) if r.size else 0.0
 
 
+def norm_rel(a: np.ndarray, b: np.ndarray) -> float:
+    """Per-field relative error normalised by the field's magnitude:
+    max|a-b| / max(max|a|, max|b|) (0 for two zero fields, inf for a NaN in
+    one of them only).  Used for fields that cross zero (velocities, fluxes),
+    where an element-wise ratio is dominated by cancellation."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if np.any(np.isnan(a) != np.isnan(b)):
+        return float("inf")
+    ok = ~np.isnan(a)
+    if not ok.any():
+        return 0.0
+    d = np.abs(a[ok] - b[ok]).max()
+    s = max(np.abs(a[ok]).max(), np.abs(b[ok]).max())
+    return 0.0 if d == 0 else float(d / s)
+
+
 def bit_equal(a: np.ndarray, b: np.ndarray) -> bool:
     """Equal as numbers (+0 == -0), with NaN == NaN."""
     a = np.asarray(a)

@@ -1,6 +1,8 @@
 """Parity of the HIP backend with the reference, through the C ABI on device 0.
 
-Bars (per field, max relative error |a-b|/max(|a|,|b|)):
+Bars, per field: the relative error max|a-b| / max(|a|,|b|) taken over the
+field (normalised by the field's magnitude), plus the element-wise ratio for
+the state fields P_w, S_w, S_n, which never approach zero:
   * the Corey closure has no transcendental calls, so GPU and oracle must
     agree bit for bit after any number of steps;
   * the reference closures call pow/exp, which the GPU takes from the ROCm
@@ -16,7 +18,7 @@ import numpy as np
 import pytest
 
 from capaz_amd import capaz, workloads
-from capaz_cases import bit_equal, case, max_rel, random_state
+from capaz_cases import bit_equal, case, max_rel, norm_rel, random_state
 from golden.make_golden import ENTRIES, load
 from oracle import oracle
 
@@ -31,12 +33,18 @@ def entries_of(model):
     return [e for e in ENTRIES if model == "3phe" or e not in ("H_E_current_calculation", "E_calculation")]
 
 
+STATE = ("P_w", "S_w", "S_n")
+
+
 def check_fields(got: dict, want: dict, tol: float, label: str):
     worst = {}
     for f, w in want.items():
-        r = max_rel(got[f], w)
+        r = norm_rel(got[f], w)
         worst[f] = r
-        assert r <= tol, f"{label}: field {f} max rel {r:.3e} > {tol:g}"
+        assert r <= tol, f"{label}: field {f} relative error {r:.3e} > {tol:g}"
+        if f in STATE:
+            e = max_rel(got[f], w)
+            assert e <= tol, f"{label}: field {f} element-wise relative error {e:.3e} > {tol:g}"
     return worst
 
 
@@ -153,13 +161,14 @@ def test_full_size_mass_conservation_without_gravity():
     d = capaz.Domain(keys, st)
     for e in ("boundary_conditions", "S_calculation", "ro_P_Xi_calculation", "u_calculation", "roS_calculation"):
         d.call(e)
-    m = d.field("m")[1:-1, 1:-1, 1:-1]
+    m = d.field("m")[1:-1, 1:-1, 1:-1].astype(np.longdouble)
     for ph in ("w", "n", "g"):
-        new = d.field("roS_" + ph)[1:-1, 1:-1, 1:-1]
-        old = d.field("roS_" + ph + "_old")[1:-1, 1:-1, 1:-1]
-        total = (m * new).sum(dtype=np.float64) - (m * old).sum(dtype=np.float64)
-        moved = np.abs(m * (new - old)).sum()
-        noise = 4 * new.size * np.abs(m * new).max() * 2.0 ** -53
+        new = d.field("roS_" + ph)[1:-1, 1:-1, 1:-1].astype(np.longdouble)
+        old = d.field("roS_" + ph + "_old")[1:-1, 1:-1, 1:-1].astype(np.longdouble)
+        # extended-precision sums: the stored roS rounding is the only loss
+        total = float((m * new).sum() - (m * old).sum())
+        moved = float(np.abs(m * (new - old)).sum())
+        noise = float(4 * new.size * np.abs(m * new).max() * 2.0 ** -53)
         assert moved > 1e3 * noise, ph
         assert abs(total) <= noise, (ph, total, noise)
 
@@ -183,7 +192,7 @@ def test_face_buffers_match_reference_layout():
     """capaz_load_face / capaz_save_face use the planes and buffer layout of
     load_/save_exchange_data_part_* (cpu.cpp:769-888)."""
     import ctypes
-    import torch
+    from capaz_amd import hip
     keys = case("3ph", 6, 5, 4)
     st = random_state(keys, 3)
     d = capaz.Domain(keys, st)
@@ -192,18 +201,17 @@ def test_face_buffers_match_reference_layout():
     fid = capaz.FIELD_ID["P_w"]
     want = {(0, 0): P[:, :, 1], (0, 1): P[:, :, -2], (1, 0): P[:, 1, :], (1, 1): P[:, -2, :],
             (2, 0): P[1, :, :], (2, 1): P[-2, :, :]}
+    buf = hip.DeviceBuffer(8 * 64)
     for (axis, side), plane in want.items():
-        buf = torch.zeros(plane.size, dtype=torch.float64, device="cuda")
-        torch.cuda.synchronize()  # the context's stream does not wait for torch's
-        assert L.capaz_load_face(d._ctx, fid, axis, side, ctypes.c_void_p(buf.data_ptr())) == 0
+        assert L.capaz_load_face(d._ctx, fid, axis, side, ctypes.c_void_p(buf.address)) == 0
         d.sync()
-        # buffer index a + na*b with (a, b) = (j, k), (i, k), (i, j): plane[k-or-j, ...] flattened
-        np.testing.assert_array_equal(buf.cpu().numpy(), plane.reshape(-1))
-    src = torch.arange(6 * 5, dtype=torch.float64, device="cuda")  # z face: Nx*Ny
-    torch.cuda.synchronize()
-    assert L.capaz_save_face(d._ctx, fid, 2, 1, ctypes.c_void_p(src.data_ptr())) == 0
+        # buffer index a + na*b with (a, b) = (j, k), (i, k), (i, j)
+        np.testing.assert_array_equal(buf.download(plane.size), plane.reshape(-1))
+    buf.upload(np.arange(6 * 5, dtype=np.float64))  # z face: Nx*Ny
+    assert L.capaz_save_face(d._ctx, fid, 2, 1, ctypes.c_void_p(buf.address)) == 0
     d.sync()
-    np.testing.assert_array_equal(d.field("P_w")[-1], src.cpu().numpy().reshape(5, 6))
+    np.testing.assert_array_equal(d.field("P_w")[-1], np.arange(30.0).reshape(5, 6))
+    buf.close()
 
 
 def test_invalid_field_reports_error():

@@ -199,12 +199,12 @@ def mass_change(d, keys):
     over the interior cells."""
     for e in ("boundary_conditions", "S_calculation", "ro_P_Xi_calculation", "u_calculation"):
         d.call(e)
-    m = d.field("m")[1:-1, 1:-1, 1:-1]
+    m = d.field("m")[1:-1, 1:-1, 1:-1].astype(np.longdouble)
     d.call("roS_calculation")
     out = []
     for ph in ("w", "n", "g"):
-        new = d.field("roS_" + ph)[1:-1, 1:-1, 1:-1]
-        old = d.field("roS_" + ph + "_old")[1:-1, 1:-1, 1:-1]
+        new = d.field("roS_" + ph)[1:-1, 1:-1, 1:-1].astype(np.longdouble)
+        old = d.field("roS_" + ph + "_old")[1:-1, 1:-1, 1:-1].astype(np.longdouble)
         # rounding of the stored roS_new is the only loss: half an ulp per cell
         noise = 4 * new.size * np.abs(m * new).max() * 2.0 ** -53
         out.append(((m * new).sum() - (m * old).sum(), np.abs(m * (new - old)).sum(), noise))
