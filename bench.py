@@ -56,7 +56,19 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device 5 times (a copy of known size, to scale "
+                         "the FETCH_SIZE / WRITE_SIZE counters), then exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        from capaz_amd import hip
+        nbytes = args.calibrate_copy << 20
+        a, b = hip.DeviceBuffer(nbytes), hip.DeviceBuffer(nbytes)
+        hip.lib().hipMemset(a.ptr, 1, nbytes)
+        for _ in range(5):
+            hip._ok(hip.lib().hipMemcpy(b.ptr, a.ptr, nbytes, hip.D2D), "hipMemcpy")
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 5}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this backend runs one domain on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU slab decomposition is a design, not a build)", file=sys.stderr)

@@ -49,6 +49,8 @@ EXPORTS = {
     "capaz_boundary": (ctypes.c_int, ["ctx"]),
     "capaz_step_sequence": (ctypes.c_int, ["ctx", ctypes.c_double]),
     "capaz_step": (ctypes.c_int, ["ctx", ctypes.c_long, ctypes.c_long]),
+    "capaz_tune": (ctypes.c_int, ["ctx", ctypes.c_char_p, ctypes.c_int]),
+    "capaz_get_tuning": (ctypes.c_int, ["ctx", ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
     "capaz_sync": (ctypes.c_int, ["ctx"]),
     "capaz_stream": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_void_p)]),
     "capaz_load_face": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
@@ -223,6 +225,19 @@ class Domain:
     def run(self, nsteps: int, first_step: int = 1) -> None:
         """nsteps fused time steps (capaz_step)."""
         self._check(load().capaz_step(self._ctx, first_step, nsteps), "capaz_step")
+
+    def tune(self, **kw) -> None:
+        """Launch shape of the fused step (capaz_tune): fused_kernel, tile_y, kchunk."""
+        for k, v in kw.items():
+            self._check(load().capaz_tune(self._ctx, k.encode(), int(v)), f"capaz_tune({k}={v})")
+
+    def tuning(self) -> dict:
+        out = {}
+        for k in ("fused_kernel", "tile_y", "kchunk"):
+            v = ctypes.c_int()
+            self._check(load().capaz_get_tuning(self._ctx, k.encode(), ctypes.byref(v)), "capaz_get_tuning")
+            out[k] = v.value
+        return out
 
     def sync(self) -> None:
         self._check(load().capaz_sync(self._ctx), "capaz_sync")

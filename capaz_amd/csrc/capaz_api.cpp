@@ -36,6 +36,7 @@ struct capaz_ctx {
     bool m_checked = false; // m has been scanned for zeros since it last changed
     bool m_has_zero = false;
     int* d_flag = nullptr;
+    capaz::Tuning tuning;
     std::string err;
 };
 
@@ -317,11 +318,35 @@ int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
         io.Sn1 = c->alt[F_S_n];
         io.Pn1 = c->alt[F_P_n];
         io.Pg1 = c->alt[F_P_g];
-        CAPAZ_HIP(c, launch_fused_step(c->p, io, c->stream));
+        CAPAZ_HIP(c, launch_fused(c->p, io, c->tuning, c->stream));
         for (int f = 0; f < F_COUNT; ++f)
             if (c->primary[f]) std::swap(c->cur[f], c->alt[f]);
         c->stale = true;
     }
+    return CAPAZ_OK;
+}
+
+int capaz_tune(capaz_ctx* c, const char* key, int value) {
+    if (!c || !key) return CAPAZ_ERR_INVALID;
+    const std::string k(key);
+    if (k == "fused_kernel" && (value == 0 || value == 1))
+        c->tuning.fused_kernel = value;
+    else if (k == "tile_y" && (value == 4 || value == 8))
+        c->tuning.tile_y = value;
+    else if (k == "kchunk" && value >= 1 && value <= 1 << 20)
+        c->tuning.kchunk = value;
+    else
+        return fail(c, CAPAZ_ERR_INVALID, "capaz_tune: unknown key or value out of range");
+    return CAPAZ_OK;
+}
+
+int capaz_get_tuning(const capaz_ctx* c, const char* key, int* value) {
+    if (!c || !key || !value) return CAPAZ_ERR_INVALID;
+    const std::string k(key);
+    if (k == "fused_kernel") *value = c->tuning.fused_kernel;
+    else if (k == "tile_y") *value = c->tuning.tile_y;
+    else if (k == "kchunk") *value = c->tuning.kchunk;
+    else return CAPAZ_ERR_INVALID;
     return CAPAZ_OK;
 }
 

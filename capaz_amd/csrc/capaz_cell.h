@@ -251,6 +251,64 @@ CAPAZ_HD int reverse_matrix(double* a) {
     return 0;
 }
 
+// reverse_matrix<3> when no pivot search is needed: the same operations on
+// every entry that is not known to be exactly 0 or 1, so the result has the
+// same bits (entries known to be zero only lose the sign of a zero, which
+// never reaches a non-zero result).  Returns false, leaving `a` untouched,
+// if some pivot has |f| < 1e-8; the caller then runs the general routine.
+CAPAZ_HD bool reverse3_nopivot(double* a) {
+    const double e = 1E-8;
+    // k = 0: row 0 = [1, a01/f, a02/f | 1/f, 0, 0]
+    double f = a[0];
+    if (fabs(f) < e) return false;
+    double rf = 1. / f;
+    double r01 = exact_div(a[1], f, rf), r02 = exact_div(a[2], f, rf), r03 = rf;
+    // rows 1, 2 lose column 0: [0, b_i1, b_i2 | b_i3, delta_i1, delta_i2]
+    double f1 = a[3];
+    double b11 = a[4] - r01 * f1, b12 = a[5] - r02 * f1, b13 = 0. - r03 * f1;
+    double f2 = a[6];
+    double b21 = a[7] - r01 * f2, b22 = a[8] - r02 * f2, b23 = 0. - r03 * f2;
+    // k = 1: row 1 = [0, 1, b12/f | b13/f, 1/f, 0]
+    f = b11;
+    if (fabs(f) < e) return false;
+    rf = 1. / f;
+    double s12 = exact_div(b12, f, rf), s13 = exact_div(b13, f, rf), s14 = rf;
+    f1 = r01;  // row 0
+    double c02 = r02 - s12 * f1, c03 = r03 - s13 * f1, c04 = 0. - s14 * f1;
+    f2 = b21;  // row 2
+    double c22 = b22 - s12 * f2, c23 = b23 - s13 * f2, c24 = 0. - s14 * f2;
+    // k = 2: row 2 = [0, 0, 1 | c23/f, c24/f, 1/f]
+    f = c22;
+    if (fabs(f) < e) return false;
+    rf = 1. / f;
+    double t23 = exact_div(c23, f, rf), t24 = exact_div(c24, f, rf), t25 = rf;
+    f1 = c02;  // row 0
+    double o03 = c03 - t23 * f1, o04 = c04 - t24 * f1, o05 = 0. - t25 * f1;
+    f2 = s12;  // row 1
+    double o13 = s13 - t23 * f2, o14 = s14 - t24 * f2, o15 = 0. - t25 * f2;
+    a[0] = o03; a[1] = o04; a[2] = o05;
+    a[3] = o13; a[4] = o14; a[5] = o15;
+    a[6] = t23; a[7] = t24; a[8] = t25;
+    return true;
+}
+
+// The general routine, kept out of line on the device: it runs only when a
+// pivot is below 1e-8, and inlining it would hold its 3 x 6 matrix in
+// registers across the whole Newton loop.
+#if defined(__HIPCC__)
+__host__ __device__ __attribute__((noinline)) inline int reverse_matrix3_general(double* a) {
+    return reverse_matrix<3>(a);
+}
+#else
+inline int reverse_matrix3_general(double* a) { return reverse_matrix<3>(a); }
+#endif
+
+// reverse_matrix<3> with the no-pivot path first.
+CAPAZ_HD int reverse3(double* a) {
+    if (reverse3_nopivot(a)) return 0;
+    return reverse_matrix3_general(a);
+}
+
 // ---- Newton solves ------------------------------------------------------------------------
 
 // Two-phase fixed-iteration 2x2 Newton by Cramer's rule (two-phase.cpp:25-56).
@@ -302,7 +360,7 @@ CAPAZ_HD void newton3(const DevParams& p, double& P, double& Sw, double& Sn, dou
         dF[6] = exact_div(p.ro0_g * Sg, p.P_atm, p.r_P_atm);
         dF[7] = -gP;
         dF[8] = -gP;
-        reverse_matrix<3>(dF);
+        reverse3(dF);
         P = P - (dF[0] * F1 + dF[1] * F2 + dF[2] * F3);
         Sw = Sw - (dF[3] * F1 + dF[4] * F2 + dF[5] * F3);
         Sn = Sn - (dF[6] * F1 + dF[7] * F2 + dF[8] * F3);

@@ -1,0 +1,301 @@
+// The fused time-step kernel for gfx950: one pass over the primary state per
+// time step (two- and three-phase models without the energy equation).
+//
+// Mapping.  A workgroup owns a 64 x TY block of interior columns (x fastest,
+// so each wavefront row streams 64 consecutive doubles of every field) and a
+// chunk of planes along z, which it walks plane by plane.  For the current
+// plane the closure coefficients of each cell -- phase pressures, Xi and
+// densities, the values the upwind flux reads from a neighbour -- are computed
+// once and staged in LDS together with a one-cell halo ring, so the x and y
+// neighbours come from LDS; the z neighbours stay in registers from the
+// previous and next plane, and the z face differences (P_c - P_zm)/h are
+// carried over from the previous plane (the same bits as recomputing them).
+// After the divergence each thread solves its cell's Newton system in
+// registers and writes the new state.  The boundary cells are written
+// afterwards by k_fused_faces from their inward neighbours' new state
+// (Border_S/Border_P, two-phase.cpp:163-215, three-phase.cpp:345-427): an
+// O(N^2) pass that keeps the face logic out of the interior kernel's
+// registers.
+//
+// Arithmetic is that of capaz_cell.h / capaz_bodies.h, so the kernel gives
+// the entry-point sequence's bits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "capaz_bodies.h"
+
+namespace capaz {
+
+template <int NPH>
+struct CoefT {
+    double P[NPH], Xi[NPH], ro[NPH];
+};
+
+// assign_P_Xi + assign_ro of one cell with the model and closure fixed at
+// compile time (cpu.cpp:143-169; two-phase.cpp:4-23; three-phase.cpp:198-240).
+template <int MODEL, int CLOSURE>
+__device__ __forceinline__ CoefT<MODEL> coef_at(const DevParams& p, const StepIO& io, int64_t l, bool inner) {
+    CoefT<MODEL> c;
+    const double Pw = io.Pw[l], K = io.K[l];
+    if constexpr (MODEL == 2) {
+        double kw, kn, Pk;
+        if constexpr (CLOSURE == 1) {
+            double se = exact_div(1. - io.Sn[l] - p.S_wr, p.den2c, p.r_den2c);
+            kw = corey_kr(p, se);
+            kn = corey_kr(p, 1. - se);
+            Pk = 0.;
+        } else {
+            relperm2(p, io.Sn[l], kw, kn, Pk);
+        }
+        const double Pn = Pw + Pk;
+        c.P[0] = Pw;
+        c.P[1] = Pn;
+        c.Xi[0] = xi_const(K, kw, p.mu_w, p.r_mu_w);
+        c.Xi[1] = xi_const(K, kn, p.mu_n, p.r_mu_n);
+        c.ro[0] = ro_w_of(p, Pw);
+        c.ro[1] = ro_n_of(p, Pn);
+    } else {
+        double kw, kn, kg;
+        const double Sw = io.Sw[l], Sn = io.Sn[l];
+        if constexpr (CLOSURE == 1) {
+            const double Swe = exact_div(Sw - p.S_wr, p.den3, p.r_den3);
+            const double Sne = exact_div(Sn - p.S_nr, p.den3, p.r_den3);
+            const double Sge = 1. - Swe - Sne;
+            kw = corey_kr(p, Swe);
+            kn = corey_kr(p, Sne);
+            kg = corey_kr(p, Sge);
+        } else {
+            relperm3(p, Sw, Sn, kw, kn, kg);
+        }
+        const double Pn = inner ? Pw : io.Pn[l];
+        const double Pg = inner ? Pw : io.Pg[l];
+        c.P[0] = Pw;
+        c.P[1] = Pn;
+        c.P[2] = Pg;
+        c.Xi[0] = xi_const(K, kw, p.mu_w, p.r_mu_w);
+        c.Xi[1] = xi_const(K, kn, p.mu_n, p.r_mu_n);
+        c.Xi[2] = xi_const(K, kg, p.mu_g, p.r_mu_g);
+        c.ro[0] = ro_w_of(p, Pw);
+        c.ro[1] = ro_n_of(p, Pn);
+        c.ro[2] = ro_g_of(p, Pg);
+    }
+    return c;
+}
+
+template <int MODEL>
+__device__ __forceinline__ bool inner_cell(const DevParams& p, int i, int j, int k) {
+    return MODEL == 2 ? interior_newton2(p, i, j, k) : interior3(p, i, j, k);
+}
+
+// Boundary cell (bi, bj, bk) from its inward neighbour's new state.
+template <int MODEL>
+__device__ __forceinline__ void write_boundary(const DevParams& p, const StepIO& io, int bi, int bj, int bk,
+                                               double Pw, double Sw, double Sn, double ro_eff) {
+    const int64_t lb = (int64_t)bi + (int64_t)bj * p.sy + (int64_t)bk * p.sz;
+    if constexpr (MODEL == 2) {
+        double Snb = Sn;
+        if (bj == 0 && p.source > 0 && in_source_window(p, bi, bk)) Snb = p.S_n_gr;
+        io.Sn1[lb] = Snb;
+        double Pb;
+        if ((bj != 0) && (bj != p.Ny - 1))
+            Pb = Pw;
+        else if (bj == 0)
+            Pb = p.P_atm;
+        else
+            Pb = Pw + ro_eff;
+        io.Pw1[lb] = Pb;
+    } else {
+        if (bj == 0 && p.source > 0) {
+            io.Sw1[lb] = p.S_w_gr;
+            io.Sn1[lb] = p.S_n_gr;
+        } else {
+            io.Sw1[lb] = Sw;
+            io.Sn1[lb] = Sn;
+        }
+        double Pwb, Pnb, Pgb;
+        border_p3(p, bj, Pw, Pwb, Pnb, Pgb);
+        io.Pw1[lb] = Pwb;
+        io.Pn1[lb] = Pnb;
+        io.Pg1[lb] = Pgb;
+    }
+}
+
+template <int NV, int TY>
+struct FusedLds {
+    double v[NV][TY + 2][66];
+};
+
+template <int MODEL, int CLOSURE, int TY>
+__global__ void __launch_bounds__(64 * TY) k_fused_z(DevParams p, StepIO io, int kchunk) {
+    constexpr int NPH = MODEL;
+    constexpr int NV = 3 * NPH;
+    __shared__ FusedLds<NV, TY> lds;
+
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const bool hasx = p.Nx >= 3, hasz = p.Nz >= 3;
+    const int ilo = hasx ? 1 : 0, ihi = hasx ? p.Nx - 2 : 0;
+    const int klo = hasz ? 1 : 0, khi = hasz ? p.Nz - 2 : 0;
+    const int i0 = ilo + 64 * (int)blockIdx.x, j0 = 1 + TY * (int)blockIdx.y;
+    const int k0 = klo + kchunk * (int)blockIdx.z;
+    const int k1 = min(khi, k0 + kchunk - 1);
+    const int i = i0 + tx, j = j0 + ty;
+    const bool own_in_grid = i < p.Nx && j < p.Ny;
+    const bool active = i <= ihi && j <= p.Ny - 2;
+
+    // This thread's halo cell of each plane, if any (ty 0 / 1: the y rows
+    // below / above the block, ty 2: the x columns left / right of it).
+    int hi = -1, hj = -1, hx = 0, hy = 0;
+    if (ty == 0) { hi = i; hj = j0 - 1; hx = tx + 1; hy = 0; }
+    else if (ty == 1) { hi = i; hj = j0 + TY; hx = tx + 1; hy = TY + 1; }
+    else if (ty == 2 && tx < TY) { hi = i0 - 1; hj = j0 + tx; hx = 0; hy = tx + 1; }
+    else if (ty == 2 && tx >= TY && tx < 2 * TY) { hi = i0 + 64; hj = j0 + tx - TY; hx = 65; hy = tx - TY + 1; }
+    const bool halo_in_grid = hi >= 0 && hi < p.Nx && hj >= 0 && hj < p.Ny && (hasx || hi == i0 + tx);
+
+    const int64_t col = (int64_t)i + (int64_t)j * p.sy;
+    const int64_t hcol = (int64_t)hi + (int64_t)hj * p.sy;
+
+    CoefT<NPH> zm{}, c{}, zp{};
+    double dz[NPH];
+    if (own_in_grid) {
+        c = coef_at<MODEL, CLOSURE>(p, io, col + (int64_t)k0 * p.sz, inner_cell<MODEL>(p, i, j, k0));
+        if (hasz) {
+            zm = coef_at<MODEL, CLOSURE>(p, io, col + (int64_t)(k0 - 1) * p.sz, inner_cell<MODEL>(p, i, j, k0 - 1));
+#pragma unroll
+            for (int a = 0; a < NPH; ++a) dz[a] = exact_div(c.P[a] - zm.P[a], p.hz, p.r_hz);
+        }
+    }
+
+    for (int k = k0; k <= k1; ++k) {
+        // stage plane k: own cell and halo cell
+        if (own_in_grid) {
+#pragma unroll
+            for (int a = 0; a < NPH; ++a) {
+                lds.v[a][ty + 1][tx + 1] = c.P[a];
+                lds.v[NPH + a][ty + 1][tx + 1] = c.Xi[a];
+                lds.v[2 * NPH + a][ty + 1][tx + 1] = c.ro[a];
+            }
+        }
+        if (hi >= 0 && halo_in_grid) {
+            CoefT<NPH> h = coef_at<MODEL, CLOSURE>(p, io, hcol + (int64_t)k * p.sz, inner_cell<MODEL>(p, hi, hj, k));
+#pragma unroll
+            for (int a = 0; a < NPH; ++a) {
+                lds.v[a][hy][hx] = h.P[a];
+                lds.v[NPH + a][hy][hx] = h.Xi[a];
+                lds.v[2 * NPH + a][hy][hx] = h.ro[a];
+            }
+        }
+        __syncthreads();
+
+        if (own_in_grid && hasz)
+            zp = coef_at<MODEL, CLOSURE>(p, io, col + (int64_t)(k + 1) * p.sz, inner_cell<MODEL>(p, i, j, k + 1));
+        double dzp[NPH];
+#pragma unroll
+        for (int a = 0; a < NPH; ++a) dzp[a] = hasz ? exact_div(zp.P[a] - c.P[a], p.hz, p.r_hz) : 0.;
+
+        if (active) {
+            const int64_t l = col + (int64_t)k * p.sz;
+            double div[NPH];
+#pragma unroll
+            for (int a = 0; a < NPH; ++a) {
+                // order as cpu.cpp:651-708: z, x, y, then (-q + fx + fy + fz)
+                double fz = 0., fx = 0.;
+                if (hasz)
+                    fz = flux_x(dz[a], dzp[a], c.Xi[a], c.ro[a], zm.Xi[a], zm.ro[a], zp.Xi[a], zp.ro[a], p.hz, p.r_hz);
+                if (hasx) {
+                    const double Pm = lds.v[a][ty + 1][tx], Pp = lds.v[a][ty + 1][tx + 2];
+                    const double dm = exact_div(c.P[a] - Pm, p.hx, p.r_hx);
+                    const double dp = exact_div(Pp - c.P[a], p.hx, p.r_hx);
+                    fx = flux_x(dm, dp, c.Xi[a], c.ro[a], lds.v[NPH + a][ty + 1][tx], lds.v[2 * NPH + a][ty + 1][tx],
+                                lds.v[NPH + a][ty + 1][tx + 2], lds.v[2 * NPH + a][ty + 1][tx + 2], p.hx, p.r_hx);
+                }
+                const double Pm = lds.v[a][ty][tx + 1], Pp = lds.v[a][ty + 2][tx + 1];
+                const double dm = exact_div(c.P[a] - Pm, p.hy, p.r_hy);
+                const double dp = exact_div(Pp - c.P[a], p.hy, p.r_hy);
+                const double fy = flux_y(p, dm, dp, c.Xi[a], c.ro[a], lds.v[NPH + a][ty][tx + 1],
+                                         lds.v[2 * NPH + a][ty][tx + 1], lds.v[NPH + a][ty + 2][tx + 1],
+                                         lds.v[2 * NPH + a][ty + 2][tx + 1]);
+                double s = (a == 2 ? 0. + fx : fx) + fy;
+                div[a] = s + fz;
+            }
+            const double m = io.m[l];
+            const double dtm = p.dt / m;
+            double Pw = c.P[0], Sn = io.Sn[l], Sw = 0.;
+            if constexpr (MODEL == 2) {
+                const double A1 = c.ro[0] * (1. - Sn) - dtm * div[0];
+                const double A2 = c.ro[1] * Sn - dtm * div[1];
+                newton2(p, Pw, Sn, A1, A2);
+                io.Pw1[l] = Pw;
+                io.Sn1[l] = Sn;
+            } else {
+                Sw = io.Sw[l];
+                const double A1 = c.ro[0] * Sw - dtm * div[0];
+                const double A2 = c.ro[1] * Sn - dtm * div[1];
+                const double A3 = c.ro[2] * (1. - Sw - Sn) - dtm * div[2];
+                newton3(p, Pw, Sw, Sn, A1, A2, A3);
+                io.Pw1[l] = Pw;
+                io.Sw1[l] = Sw;
+                io.Sn1[l] = Sn;
+            }
+        }
+        __syncthreads();
+        // roll the z window
+#pragma unroll
+        for (int a = 0; a < NPH; ++a) {
+            dz[a] = dzp[a];
+            zm.Xi[a] = c.Xi[a];
+            zm.ro[a] = c.ro[a];
+        }
+        c = zp;
+    }
+}
+
+// Boundary cells after k_fused_z: thread t enumerates the j faces (all i, k),
+// then the x faces (j interior, all k), then the z faces (i, j interior).
+template <int MODEL, int CLOSURE>
+__global__ void __launch_bounds__(256) k_fused_faces(DevParams p, StepIO io) {
+    const bool hasx = p.Nx >= 3, hasz = p.Nz >= 3;
+    const int64_t nj = 2LL * p.Nx * p.Nz;
+    const int64_t nx = hasx ? 2LL * (p.Ny - 2) * p.Nz : 0;
+    const int ni = hasx ? p.Nx - 2 : p.Nx;
+    const int64_t nzf = hasz ? 2LL * ni * (p.Ny - 2) : 0;
+    int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int i, j, k;
+    if (t < nj) {
+        j = (t & 1) ? p.Ny - 1 : 0;
+        t >>= 1;
+        i = (int)(t % p.Nx);
+        k = (int)(t / p.Nx);
+    } else if ((t -= nj) < nx) {
+        i = (t & 1) ? p.Nx - 1 : 0;
+        t >>= 1;
+        j = 1 + (int)(t % (p.Ny - 2));
+        k = (int)(t / (p.Ny - 2));
+    } else if ((t -= nx) < nzf) {
+        k = (t & 1) ? p.Nz - 1 : 0;
+        t >>= 1;
+        i = (hasx ? 1 : 0) + (int)(t % ni);
+        j = 1 + (int)(t / ni);
+    } else {
+        return;
+    }
+    const int64_t l1 = inward(p, i, j, k);
+    const double Pw = io.Pw1[l1], Sn = io.Sn1[l1];
+    const double Sw = MODEL == 3 ? io.Sw1[l1] : 0.;
+    double ro_eff = 0.;
+    if constexpr (MODEL == 2) {
+        // ro_eff_gdy (main.cpp:255-267): this step's densities of the inward
+        // cell (from its previous state) with its new saturation
+        const CoefT<2> c1 = coef_at<2, CLOSURE>(p, io, l1, true);
+        ro_eff = (c1.ro[1] * Sn + c1.ro[0] * (1 - Sn)) * io.m[l1] * p.g * p.hy;
+    }
+    write_boundary<MODEL>(p, io, i, j, k, Pw, Sw, Sn, ro_eff);
+}
+
+inline int64_t fused_face_cells(const DevParams& p) {
+    const bool hasx = p.Nx >= 3, hasz = p.Nz >= 3;
+    const int64_t ni = hasx ? p.Nx - 2 : p.Nx;
+    return 2LL * p.Nx * p.Nz + (hasx ? 2LL * (p.Ny - 2) * p.Nz : 0) + (hasz ? 2LL * ni * (p.Ny - 2) : 0);
+}
+
+}  // namespace capaz

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "capaz_bodies.h"
+#include "capaz_fused.h"
 #include "capaz_launch.h"
 
 namespace capaz {
@@ -113,6 +114,31 @@ hipError_t launch_entry(Entry e, const DevParams& p, const DevFields& F, hipStre
 hipError_t launch_fused_step(const DevParams& p, const StepIO& io, hipStream_t s) {
     hipLaunchKernelGGL(k_fused_step, grid_for(p.n), dim3(kBlock), 0, s, p, io);
     return hipGetLastError();
+}
+
+template <int MODEL, int CLOSURE, int TY>
+hipError_t launch_z(const DevParams& p, const StepIO& io, int kchunk, hipStream_t s) {
+    const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
+    if (kchunk < 1) kchunk = 1;
+    dim3 grid((nx + 63) / 64, (ny + TY - 1) / TY, (nz + kchunk - 1) / kchunk);
+    hipLaunchKernelGGL((k_fused_z<MODEL, CLOSURE, TY>), grid, dim3(64, TY), 0, s, p, io, kchunk);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_fused_faces<MODEL, CLOSURE>), grid_for(fused_face_cells(p)), dim3(256), 0, s, p, io);
+    return hipGetLastError();
+}
+
+template <int MODEL, int CLOSURE>
+hipError_t launch_z_ty(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s) {
+    if (t.tile_y == 8) return launch_z<MODEL, CLOSURE, 8>(p, io, t.kchunk, s);
+    return launch_z<MODEL, CLOSURE, 4>(p, io, t.kchunk, s);
+}
+
+hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s) {
+    if (t.fused_kernel == 0) return launch_fused_step(p, io, s);
+    if (p.model == 3)
+        return p.closure == 1 ? launch_z_ty<3, 1>(p, io, t, s) : launch_z_ty<3, 0>(p, io, t, s);
+    return p.closure == 1 ? launch_z_ty<2, 1>(p, io, t, s) : launch_z_ty<2, 0>(p, io, t, s);
 }
 
 hipError_t launch_copy_interior(const DevParams& p, const double* src, double* dst, hipStream_t s) {

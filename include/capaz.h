@@ -156,6 +156,15 @@ int capaz_step_sequence(capaz_ctx* ctx, double t);
  * as capaz_step_sequence, bit for bit, in fewer passes over memory. */
 int capaz_step(capaz_ctx* ctx, long first_step, long nsteps);
 
+/* Launch shape of capaz_step's kernels (does not change results):
+ *   "fused_kernel"  0: one thread per cell; 1: z-marching 64 x tile_y tiles
+ *                   with the plane's coefficients staged in LDS (default)
+ *   "tile_y"        4 or 8 (default 4)
+ *   "kchunk"        planes per workgroup along z (default 16)
+ * Returns CAPAZ_ERR_INVALID for an unknown key or value. */
+int capaz_tune(capaz_ctx* ctx, const char* key, int value);
+int capaz_get_tuning(const capaz_ctx* ctx, const char* key, int* value);
+
 /* Waits for all work queued on the context. */
 int capaz_sync(capaz_ctx* ctx);
 
