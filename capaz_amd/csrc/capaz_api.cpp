@@ -36,6 +36,7 @@ struct capaz_ctx {
     bool m_checked = false; // m has been scanned for zeros since it last changed
     bool m_has_zero = false;
     int* d_flag = nullptr;
+    double* d_dtm = nullptr;  // dt / m per cell, refreshed with the porosity scan
     capaz::Tuning tuning;
     std::string err;
 };
@@ -127,6 +128,7 @@ int scan_m(capaz_ctx* c) {
     if (c->m_checked) return CAPAZ_OK;
     CAPAZ_HIP(c, hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
     CAPAZ_HIP(c, capaz::launch_m_zero(c->p, c->cur[capaz::F_m], c->d_flag, c->stream));
+    CAPAZ_HIP(c, capaz::launch_dtm(c->p, c->cur[capaz::F_m], c->d_dtm, c->stream));
     int h = 0;
     CAPAZ_HIP(c, hipMemcpyAsync(&h, c->d_flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     CAPAZ_HIP(c, hipStreamSynchronize(c->stream));
@@ -172,6 +174,7 @@ int capaz_create(const capaz_params* u, capaz_ctx** out) {
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bail(e, "hipStreamCreate");
     if ((e = hipMalloc(&c->d_flag, sizeof(int))) != hipSuccess) return bail(e, "hipMalloc");
     const size_t bytes = (size_t)c->p.n * sizeof(double);
+    if ((e = hipMalloc(&c->d_dtm, bytes)) != hipSuccess) return bail(e, "hipMalloc");
     for (int f = 0; f < capaz::F_COUNT; ++f) {
         if (!field_exists(*u, f)) continue;
         if ((e = hipMalloc(&c->cur[f], bytes)) != hipSuccess) return bail(e, "hipMalloc");
@@ -196,6 +199,7 @@ int capaz_destroy(capaz_ctx* c) {
         if (c->alt[f]) (void)hipFree(c->alt[f]);
     }
     if (c->d_flag) (void)hipFree(c->d_flag);
+    if (c->d_dtm) (void)hipFree(c->d_dtm);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return CAPAZ_OK;
@@ -313,6 +317,7 @@ int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
         io.Pg = c->cur[F_P_g];
         io.m = c->cur[F_m];
         io.K = c->cur[F_K];
+        io.dtm = c->d_dtm;
         io.Pw1 = c->alt[F_P_w];
         io.Sw1 = c->alt[F_S_w];
         io.Sn1 = c->alt[F_S_n];
@@ -329,7 +334,7 @@ int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
 int capaz_tune(capaz_ctx* c, const char* key, int value) {
     if (!c || !key) return CAPAZ_ERR_INVALID;
     const std::string k(key);
-    if (k == "fused_kernel" && (value == 0 || value == 1))
+    if (k == "fused_kernel" && (value >= 0 && value <= 2))
         c->tuning.fused_kernel = value;
     else if (k == "tile_y" && (value == 4 || value == 8))
         c->tuning.tile_y = value;

@@ -314,6 +314,7 @@ struct Coef {
 
 struct StepIO {
     const double *Pw, *Sw, *Sn, *Pn, *Pg, *m, *K;  // current state (Sw unused for two phases)
+    const double* dtm;                             // dt / m per cell (fused z kernel)
     double *Pw1, *Sw1, *Sn1, *Pn1, *Pg1;           // next state
 };
 

@@ -30,7 +30,19 @@ namespace capaz {
 template <int NPH>
 struct CoefT {
     double P[NPH], Xi[NPH], ro[NPH];
+    double Sw, Sn;  // the cell's saturations, as read
 };
+
+// Corey k_r with the exponent fixed at 2 (CLOSURE 2) or read from p (1).
+template <int CLOSURE>
+__device__ __forceinline__ double corey_t(const DevParams& p, double s) {
+    if constexpr (CLOSURE == 2) {
+        s = clamp01(s);
+        return s * s;
+    } else {
+        return corey_kr(p, s);
+    }
+}
 
 // assign_P_Xi + assign_ro of one cell with the model and closure fixed at
 // compile time (cpu.cpp:143-169; two-phase.cpp:4-23; three-phase.cpp:198-240).
@@ -40,13 +52,16 @@ __device__ __forceinline__ CoefT<MODEL> coef_at(const DevParams& p, const StepIO
     const double Pw = io.Pw[l], K = io.K[l];
     if constexpr (MODEL == 2) {
         double kw, kn, Pk;
-        if constexpr (CLOSURE == 1) {
-            double se = exact_div(1. - io.Sn[l] - p.S_wr, p.den2c, p.r_den2c);
-            kw = corey_kr(p, se);
-            kn = corey_kr(p, 1. - se);
+        const double Sn = io.Sn[l];
+        c.Sn = Sn;
+        c.Sw = 0.;
+        if constexpr (CLOSURE >= 1) {
+            double se = exact_div(1. - Sn - p.S_wr, p.den2c, p.r_den2c);
+            kw = corey_t<CLOSURE>(p, se);
+            kn = corey_t<CLOSURE>(p, 1. - se);
             Pk = 0.;
         } else {
-            relperm2(p, io.Sn[l], kw, kn, Pk);
+            relperm2(p, Sn, kw, kn, Pk);
         }
         const double Pn = Pw + Pk;
         c.P[0] = Pw;
@@ -58,13 +73,15 @@ __device__ __forceinline__ CoefT<MODEL> coef_at(const DevParams& p, const StepIO
     } else {
         double kw, kn, kg;
         const double Sw = io.Sw[l], Sn = io.Sn[l];
-        if constexpr (CLOSURE == 1) {
+        c.Sw = Sw;
+        c.Sn = Sn;
+        if constexpr (CLOSURE >= 1) {
             const double Swe = exact_div(Sw - p.S_wr, p.den3, p.r_den3);
             const double Sne = exact_div(Sn - p.S_nr, p.den3, p.r_den3);
             const double Sge = 1. - Swe - Sne;
-            kw = corey_kr(p, Swe);
-            kn = corey_kr(p, Sne);
-            kg = corey_kr(p, Sge);
+            kw = corey_t<CLOSURE>(p, Swe);
+            kn = corey_t<CLOSURE>(p, Sne);
+            kg = corey_t<CLOSURE>(p, Sge);
         } else {
             relperm3(p, Sw, Sn, kw, kn, kg);
         }
@@ -126,7 +143,10 @@ struct FusedLds {
     double v[NV][TY + 2][66];
 };
 
-template <int MODEL, int CLOSURE, int TY>
+// CARRY: keep the next plane's coefficients of the own column in registers
+// (computed for the z flux) as the next centre, or recompute the centre at
+// the top of every plane (fewer live registers across the Newton solve).
+template <int MODEL, int CLOSURE, int TY, bool CARRY>
 __global__ void __launch_bounds__(64 * TY) k_fused_z(DevParams p, StepIO io, int kchunk) {
     constexpr int NPH = MODEL;
     constexpr int NV = 3 * NPH;
@@ -167,6 +187,8 @@ __global__ void __launch_bounds__(64 * TY) k_fused_z(DevParams p, StepIO io, int
     }
 
     for (int k = k0; k <= k1; ++k) {
+        if (!CARRY && own_in_grid && k != k0)
+            c = coef_at<MODEL, CLOSURE>(p, io, col + (int64_t)k * p.sz, inner_cell<MODEL>(p, i, j, k));
         // stage plane k: own cell and halo cell
         if (own_in_grid) {
 #pragma unroll
@@ -218,9 +240,8 @@ __global__ void __launch_bounds__(64 * TY) k_fused_z(DevParams p, StepIO io, int
                 double s = (a == 2 ? 0. + fx : fx) + fy;
                 div[a] = s + fz;
             }
-            const double m = io.m[l];
-            const double dtm = p.dt / m;
-            double Pw = c.P[0], Sn = io.Sn[l], Sw = 0.;
+            const double dtm = io.dtm[l];  // dt / m, computed once per porosity field
+            double Pw = c.P[0], Sn = c.Sn, Sw = c.Sw;
             if constexpr (MODEL == 2) {
                 const double A1 = c.ro[0] * (1. - Sn) - dtm * div[0];
                 const double A2 = c.ro[1] * Sn - dtm * div[1];
@@ -228,7 +249,6 @@ __global__ void __launch_bounds__(64 * TY) k_fused_z(DevParams p, StepIO io, int
                 io.Pw1[l] = Pw;
                 io.Sn1[l] = Sn;
             } else {
-                Sw = io.Sw[l];
                 const double A1 = c.ro[0] * Sw - dtm * div[0];
                 const double A2 = c.ro[1] * Sn - dtm * div[1];
                 const double A3 = c.ro[2] * (1. - Sw - Sn) - dtm * div[2];
@@ -246,7 +266,7 @@ __global__ void __launch_bounds__(64 * TY) k_fused_z(DevParams p, StepIO io, int
             zm.Xi[a] = c.Xi[a];
             zm.ro[a] = c.ro[a];
         }
-        c = zp;
+        if (CARRY) c = zp;
     }
 }
 

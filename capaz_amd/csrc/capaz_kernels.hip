@@ -61,6 +61,13 @@ __global__ void __launch_bounds__(kBlock) k_copy_interior(DevParams p, const dou
     if (interior3(p, i, j, k)) dst[l] = src[l];
 }
 
+// dt / m of every cell, the factor of the mass update (cpu.cpp:707-708, 723),
+// computed once when the porosity changes instead of once per step.
+__global__ void __launch_bounds__(kBlock) k_dtm(DevParams p, const double* m, double* dtm) {
+    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l < p.n) dtm[l] = p.dt / m[l];
+}
+
 // Porosity check of the fused path: sets *flag if an interior cell has m == 0
 // (assign_roS_nr skips such cells, cpu.cpp:625-626).
 __global__ void __launch_bounds__(kBlock) k_m_zero(DevParams p, const double* m, int* flag) {
@@ -116,12 +123,12 @@ hipError_t launch_fused_step(const DevParams& p, const StepIO& io, hipStream_t s
     return hipGetLastError();
 }
 
-template <int MODEL, int CLOSURE, int TY>
+template <int MODEL, int CLOSURE, int TY, bool CARRY>
 hipError_t launch_z(const DevParams& p, const StepIO& io, int kchunk, hipStream_t s) {
     const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
     if (kchunk < 1) kchunk = 1;
     dim3 grid((nx + 63) / 64, (ny + TY - 1) / TY, (nz + kchunk - 1) / kchunk);
-    hipLaunchKernelGGL((k_fused_z<MODEL, CLOSURE, TY>), grid, dim3(64, TY), 0, s, p, io, kchunk);
+    hipLaunchKernelGGL((k_fused_z<MODEL, CLOSURE, TY, CARRY>), grid, dim3(64, TY), 0, s, p, io, kchunk);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_fused_faces<MODEL, CLOSURE>), grid_for(fused_face_cells(p)), dim3(256), 0, s, p, io);
@@ -130,15 +137,31 @@ hipError_t launch_z(const DevParams& p, const StepIO& io, int kchunk, hipStream_
 
 template <int MODEL, int CLOSURE>
 hipError_t launch_z_ty(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s) {
-    if (t.tile_y == 8) return launch_z<MODEL, CLOSURE, 8>(p, io, t.kchunk, s);
-    return launch_z<MODEL, CLOSURE, 4>(p, io, t.kchunk, s);
+    const bool carry = t.fused_kernel == 1;
+    if (t.tile_y == 8)
+        return carry ? launch_z<MODEL, CLOSURE, 8, true>(p, io, t.kchunk, s)
+                     : launch_z<MODEL, CLOSURE, 8, false>(p, io, t.kchunk, s);
+    return carry ? launch_z<MODEL, CLOSURE, 4, true>(p, io, t.kchunk, s)
+                 : launch_z<MODEL, CLOSURE, 4, false>(p, io, t.kchunk, s);
+}
+
+// CLOSURE ids of the fused kernels: 0 reference, 1 Corey with a run-time
+// exponent, 2 Corey with exponent 2 (no pow in the kernel).
+template <int MODEL>
+hipError_t launch_z_model(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s) {
+    if (p.closure == 0) return launch_z_ty<MODEL, 0>(p, io, t, s);
+    if (p.corey_n == 2.) return launch_z_ty<MODEL, 2>(p, io, t, s);
+    return launch_z_ty<MODEL, 1>(p, io, t, s);
 }
 
 hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s) {
     if (t.fused_kernel == 0) return launch_fused_step(p, io, s);
-    if (p.model == 3)
-        return p.closure == 1 ? launch_z_ty<3, 1>(p, io, t, s) : launch_z_ty<3, 0>(p, io, t, s);
-    return p.closure == 1 ? launch_z_ty<2, 1>(p, io, t, s) : launch_z_ty<2, 0>(p, io, t, s);
+    return p.model == 3 ? launch_z_model<3>(p, io, t, s) : launch_z_model<2>(p, io, t, s);
+}
+
+hipError_t launch_dtm(const DevParams& p, const double* m, double* dtm, hipStream_t s) {
+    hipLaunchKernelGGL(k_dtm, grid_for(p.n), dim3(kBlock), 0, s, p, m, dtm);
+    return hipGetLastError();
 }
 
 hipError_t launch_copy_interior(const DevParams& p, const double* src, double* dst, hipStream_t s) {
