@@ -303,10 +303,18 @@ __host__ __device__ __attribute__((noinline)) inline int reverse_matrix3_general
 inline int reverse_matrix3_general(double* a) { return reverse_matrix<3>(a); }
 #endif
 
-// reverse_matrix<3> with the no-pivot path first.
+// reverse_matrix<3> with the no-pivot path first.  `a` stays in registers;
+// only the rare fallback copies it to a stack buffer for the out-of-line call
+// (reverse3_nopivot leaves `a` unchanged when it fails).
 CAPAZ_HD int reverse3(double* a) {
     if (reverse3_nopivot(a)) return 0;
-    return reverse_matrix3_general(a);
+    double buf[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) buf[i] = a[i];
+    const int rc = reverse_matrix3_general(buf);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) a[i] = buf[i];
+    return rc;
 }
 
 // ---- Newton solves ------------------------------------------------------------------------

@@ -15,7 +15,7 @@ enum class Entry { S, P_Xi_ro, U, RoS, HE, E, Newton, Boundary };
 // in registers, 2 = the same tiles recomputing them per plane.
 struct Tuning {
     int fused_kernel = 1;
-    int tile_y = 4;
+    int tile_y = 8;
     int kchunk = 16;
 };
 

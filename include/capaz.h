@@ -160,7 +160,7 @@ int capaz_step(capaz_ctx* ctx, long first_step, long nsteps);
  *   "fused_kernel"  0: one thread per cell; 1: z-marching 64 x tile_y tiles
  *                   with the plane's coefficients staged in LDS (default);
  *                   2: as 1, recomputing the centre coefficients per plane
- *   "tile_y"        4 or 8 (default 4)
+ *   "tile_y"        4 or 8 (default 8)
  *   "kchunk"        planes per workgroup along z (default 16)
  * Returns CAPAZ_ERR_INVALID for an unknown key or value. */
 int capaz_tune(capaz_ctx* ctx, const char* key, int value);
