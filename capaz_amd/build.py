@@ -5,6 +5,8 @@
 * ``tests/emul/libcapaz_emul.so`` -- the same per-cell kernel bodies compiled
   for the host by g++, so tests can check the kernels' arithmetic against the
   oracle without a GPU (test infrastructure; never loaded by the product).
+* ``tests/hip/libcapaz_selftest.so`` -- device checks of the kernels' exact
+  arithmetic rewrites against IEEE operations (test infrastructure).
 
 All code is compiled without floating-point contraction: the kernels must
 reproduce the reference's IEEE operation sequence, and the only FMAs are the
@@ -23,6 +25,8 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libcapaz_amd.so")
 EMUL_SRC = os.path.join(ROOT, "tests", "emul", "capaz_emul.cpp")
 EMUL_LIB = os.path.join(ROOT, "tests", "emul", "libcapaz_emul.so")
+SELFTEST_SRC = os.path.join(ROOT, "tests", "hip", "capaz_selftest.hip")
+SELFTEST_LIB = os.path.join(ROOT, "tests", "hip", "libcapaz_selftest.so")
 
 ARCH = os.environ.get("CAPAZ_OFFLOAD_ARCH", "gfx950")
 
@@ -67,10 +71,19 @@ def build_emulator() -> str:
     return EMUL_LIB
 
 
+def build_selftest() -> str:
+    tmp = SELFTEST_LIB + ".tmp"
+    _run([_hipcc(), *HIP_FLAGS, "-I", CSRC, "-x", "hip", SELFTEST_SRC, "-o", tmp])
+    os.replace(tmp, SELFTEST_LIB)
+    return SELFTEST_LIB
+
+
 def build_all() -> None:
     build_library()
     if os.path.exists(EMUL_SRC):
         build_emulator()
+    if os.path.exists(SELFTEST_SRC):
+        build_selftest()
 
 
 if __name__ == "__main__":

@@ -41,6 +41,33 @@ CAPAZ_HD double exact_div(double a, double b, double rb) {
     return fma(r, rb, q);
 }
 
+// RN(1/b): the hardware reciprocal estimate, two Newton-Raphson steps and a
+// final residual correction (Markstein: the correction of an estimate within
+// one ulp of 1/b rounds to RN(1/b)).  Operands outside [2^-1000, 2^1000] in
+// magnitude, zeros and non-finite values take the IEEE division, out of line
+// so that the common path is not flattened with it.  Checked against 1.0/b
+// on the device by tests/test_selftest.py.
+#if defined(__HIPCC__)
+__device__ __attribute__((noinline)) inline double rcp_ieee(double b) { return 1. / b; }
+#endif
+
+CAPAZ_HD double rcp_rn(double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double ab = fabs(b);
+    double y = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, y, 1.);
+    y = fma(e, y, y);
+    e = fma(-b, y, 1.);
+    y = fma(e, y, y);
+    e = fma(-b, y, 1.);
+    y = fma(e, y, y);
+    if (__builtin_expect(!(ab >= 0x1p-1000 && ab <= 0x1p1000), 0)) y = rcp_ieee(b);
+    return y;
+#else
+    return 1. / b;
+#endif
+}
+
 CAPAZ_HD double half(double x) { return x * 0.5; }
 CAPAZ_HD double clamp01(double x) { return x < 0. ? 0. : (x > 1. ? 1. : x); }
 
@@ -261,7 +288,7 @@ CAPAZ_HD bool reverse3_nopivot(double* a) {
     // k = 0: row 0 = [1, a01/f, a02/f | 1/f, 0, 0]
     double f = a[0];
     if (fabs(f) < e) return false;
-    double rf = 1. / f;
+    double rf = rcp_rn(f);
     double r01 = exact_div(a[1], f, rf), r02 = exact_div(a[2], f, rf), r03 = rf;
     // rows 1, 2 lose column 0: [0, b_i1, b_i2 | b_i3, delta_i1, delta_i2]
     double f1 = a[3];
@@ -271,7 +298,7 @@ CAPAZ_HD bool reverse3_nopivot(double* a) {
     // k = 1: row 1 = [0, 1, b12/f | b13/f, 1/f, 0]
     f = b11;
     if (fabs(f) < e) return false;
-    rf = 1. / f;
+    rf = rcp_rn(f);
     double s12 = exact_div(b12, f, rf), s13 = exact_div(b13, f, rf), s14 = rf;
     f1 = r01;  // row 0
     double c02 = r02 - s12 * f1, c03 = r03 - s13 * f1, c04 = 0. - s14 * f1;
@@ -280,7 +307,7 @@ CAPAZ_HD bool reverse3_nopivot(double* a) {
     // k = 2: row 2 = [0, 0, 1 | c23/f, c24/f, 1/f]
     f = c22;
     if (fabs(f) < e) return false;
-    rf = 1. / f;
+    rf = rcp_rn(f);
     double t23 = exact_div(c23, f, rf), t24 = exact_div(c24, f, rf), t25 = rf;
     f1 = c02;  // row 0
     double o03 = c03 - t23 * f1, o04 = c04 - t24 * f1, o05 = 0. - t25 * f1;
@@ -303,11 +330,47 @@ __host__ __device__ __attribute__((noinline)) inline int reverse_matrix3_general
 inline int reverse_matrix3_general(double* a) { return reverse_matrix<3>(a); }
 #endif
 
+// reverse3_nopivot for a matrix whose a[2] is +0 (the three-phase Jacobian's
+// dF[2], three-phase.cpp:321): a02/f is +-0, so b12 = a12 - (+-0)*f1 = a12,
+// b22 = a22 and c02 = (+-0) - s12*r01 = 0. - s12*r01, all exact.
+CAPAZ_HD bool reverse3_nopivot_a02zero(double* a) {
+    const double e = 1E-8;
+    double f = a[0];
+    if (fabs(f) < e) return false;
+    double rf = rcp_rn(f);
+    double r01 = exact_div(a[1], f, rf), r03 = rf;
+    double f1 = a[3];
+    double b11 = a[4] - r01 * f1, b12 = a[5], b13 = 0. - r03 * f1;
+    double f2 = a[6];
+    double b21 = a[7] - r01 * f2, b22 = a[8], b23 = 0. - r03 * f2;
+    f = b11;
+    if (fabs(f) < e) return false;
+    rf = rcp_rn(f);
+    double s12 = exact_div(b12, f, rf), s13 = exact_div(b13, f, rf), s14 = rf;
+    f1 = r01;
+    double c02 = 0. - s12 * f1, c03 = r03 - s13 * f1, c04 = 0. - s14 * f1;
+    f2 = b21;
+    double c22 = b22 - s12 * f2, c23 = b23 - s13 * f2, c24 = 0. - s14 * f2;
+    f = c22;
+    if (fabs(f) < e) return false;
+    rf = rcp_rn(f);
+    double t23 = exact_div(c23, f, rf), t24 = exact_div(c24, f, rf), t25 = rf;
+    f1 = c02;
+    double o03 = c03 - t23 * f1, o04 = c04 - t24 * f1, o05 = 0. - t25 * f1;
+    f2 = s12;
+    double o13 = s13 - t23 * f2, o14 = s14 - t24 * f2, o15 = 0. - t25 * f2;
+    a[0] = o03; a[1] = o04; a[2] = o05;
+    a[3] = o13; a[4] = o14; a[5] = o15;
+    a[6] = t23; a[7] = t24; a[8] = t25;
+    return true;
+}
+
 // reverse_matrix<3> with the no-pivot path first.  `a` stays in registers;
 // only the rare fallback copies it to a stack buffer for the out-of-line call
 // (reverse3_nopivot leaves `a` unchanged when it fails).
+template <bool A02ZERO = false>
 CAPAZ_HD int reverse3(double* a) {
-    if (reverse3_nopivot(a)) return 0;
+    if (A02ZERO ? reverse3_nopivot_a02zero(a) : reverse3_nopivot(a)) return 0;
     double buf[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) buf[i] = a[i];
@@ -339,7 +402,7 @@ CAPAZ_HD void newton2(const DevParams& p, double& P, double& Sn, double roS_w, d
         double F1S = (-1) * p.ro0_w * (1 + p.beta_w * (P - p.P_atm));
         double F2S = p.ro0_n * (1 + p.beta_n * (P + P_k - p.P_atm + (Sn * PkS)));
         double det = F1P * F2S - F1S * F2P;
-        double rdet = 1 / det;
+        double rdet = rcp_rn(det);  // 1 / det
         P = P - rdet * (F2S * F1 - F1S * F2);
         Sn = Sn - rdet * (F1P * F2 - F2P * F1);
     }
@@ -368,7 +431,7 @@ CAPAZ_HD void newton3(const DevParams& p, double& P, double& Sw, double& Sn, dou
         dF[6] = exact_div(p.ro0_g * Sg, p.P_atm, p.r_P_atm);
         dF[7] = -gP;
         dF[8] = -gP;
-        reverse3(dF);
+        reverse3<true>(dF);
         P = P - (dF[0] * F1 + dF[1] * F2 + dF[2] * F3);
         Sw = Sw - (dF[3] * F1 + dF[4] * F2 + dF[5] * F3);
         Sn = Sn - (dF[6] * F1 + dF[7] * F2 + dF[8] * F3);

@@ -53,6 +53,7 @@ int capaz_emul_fused_step(const capaz_params* u, double** cur, double** next) {
 
 int capaz_emul_reverse_matrix(double* a, int n) {
     if (n == -3) return capaz::reverse3(a);  // the no-pivot fast path with fallback
+    if (n == -4) return capaz::reverse3<true>(a);  // the same for a[2] == +0
     if (n == 3) return capaz::reverse_matrix<3>(a);
     if (n == 5) return capaz::reverse_matrix<5>(a);
     return -2;

@@ -45,11 +45,12 @@ def validate(keys: dict) -> str:
     return lib().capaz_emul_validate(ctypes.byref(p)).decode()
 
 
-def reverse_matrix(a: np.ndarray, fast: bool = False) -> tuple[int, np.ndarray]:
+def reverse_matrix(a: np.ndarray, fast: bool = False, a02zero: bool = False) -> tuple[int, np.ndarray]:
     """The kernels' reverse_matrix<n>; fast=True runs the 3x3 no-pivot path
-    with its fallback (the one newton3 uses)."""
+    with its fallback, a02zero=True its variant for a[0,2] == +0 (the one
+    newton3 uses)."""
     buf = np.ascontiguousarray(a, dtype=np.float64).copy()
-    n = -3 if fast else a.shape[0]
+    n = -4 if a02zero else (-3 if fast else a.shape[0])
     rc = lib().capaz_emul_reverse_matrix(buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
     return rc, buf
 

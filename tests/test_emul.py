@@ -43,3 +43,19 @@ def test_reverse3_fast_path_matches_oracle():
         assert bit_equal(out_o, out_e), a
         hits += rc_o == 0
     assert hits > 10000
+
+
+def test_reverse3_a02zero_path_matches_oracle():
+    rng = np.random.default_rng(8)
+    hits = 0
+    for a in random_matrices(3, 20000, 9):
+        a = a.copy()
+        a[0, 2] = 0.0
+        if rng.uniform() < 0.1:
+            a[1, 2] = 0.0  # more structural zeros
+        rc_o, out_o = oracle.reverse_matrix(a)
+        rc_e, out_e = emul.reverse_matrix(a, a02zero=True)
+        assert rc_o == rc_e
+        assert bit_equal(out_o, out_e), a
+        hits += rc_o == 0
+    assert hits > 10000
