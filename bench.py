@@ -3,11 +3,11 @@
 filtration step at 256^3 on one MI355X, with the HBM roofline of the fused
 step kernel and the CPU oracle timed beside it.
 
-    python bench.py [--steps K] [--warmup W] [--n 256] [--no-cpu-baseline]
+    python bench.py [--steps K] [--warmup W] [--n 256] [--no-cpu-baseline] [--no-ensemble]
 
 A step is one fused time step (capaz_step: closures, upwind Darcy fluxes,
 mass update, 4-iteration 3x3 Newton per cell and the boundary faces) over all
-cells of the synthetic config-1 physics (capaz_amd/workloads.py).  The state
+cells of the synthetic three-phase injection workload (capaz_amd/workloads.py).  The state
 is resident in HBM before timing starts.  Prints one JSON line.
 """
 from __future__ import annotations
@@ -18,6 +18,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -27,26 +29,106 @@ FP64_PEAK_TFLOPS = 78.6
 # Compulsory traffic of one single-pass ping-pong three-phase step per cell:
 # read P_w, S_w, S_n, m, K and write P_w, S_w, S_n (8 doubles).
 ALGO_BYTES_PER_CELL = 64
+# Measured per interior cell of one main-kernel launch at 256^3: HBM bytes
+# (rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors that make
+# the counters read a device-to-device copy of known size correctly) and FP64
+# flops (64 lanes x (ADD + MUL + TRANS + 2 FMA) of SQ_INSTS_VALU_*_F64).
+# profiles/fused_step_3ph_256.md holds the runs and their commands.
+TRAFFIC_BYTES_PER_CELL = 93.2
+FP64_FLOPS_PER_CELL = 973.6
 
 
-def cpu_baseline(nz_slab: int = 64, steps: int = 2) -> dict:
-    """The oracle (C restatement of the reference CPU path, one core) on a
-    slab of the same workload: same physics and seeded fields, 256 x 256 x
-    nz_slab cells, `steps` steps."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n: int, steps: int = 3, nz_slab: int = 32, slab_steps: int = 2) -> dict:
+    """The oracle (C restatement of the reference CPU path, bit-identical to
+    it in the reference closures) timed on this host: the full n^3 headline
+    workload for `steps` steps with its per-cell passes on all OpenMP threads,
+    and a 256 x 256 x nz_slab slab on one thread.  Same physics and seeded
+    fields as the GPU run; only the step loop is timed."""
     from capaz_amd import workloads
     from oracle import oracle
 
-    keys, st = workloads.headline(256)
-    keys = dict(keys, NZ=nz_slab)
-    st = {k: v[:nz_slab].copy() for k, v in st.items()}
-    d = oracle.OracleDomain(keys, st)
+    def timed(keys, st, nsteps):
+        d = oracle.OracleDomain(keys, st)
+        t0 = time.perf_counter()
+        d.run(nsteps)
+        return time.perf_counter() - t0
+
+    keys, st = workloads.headline(n)
+    threads = oracle.threads()
+    dt_all = timed(keys, st, steps)
+    slab_keys = dict(keys, NZ=nz_slab)
+    slab = {k: v[:nz_slab].copy() for k, v in st.items()}
+    del st
+    oracle.set_threads(1)
+    try:
+        dt_one = timed(slab_keys, slab, slab_steps)
+    finally:
+        oracle.set_threads(threads)
+    one = n * n * nz_slab * slab_steps / dt_one / 1e6
+    return {"value": round(n ** 3 * steps / dt_all / 1e6, 2), "unit": "Mcell-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} steps of the {n}^3 headline workload (same physics and seeded fields), "
+                      f"oracle/liboracle.so with its per-cell passes on {threads} OpenMP threads "
+                      f"({_cpu_model()}), {dt_all:.1f} s",
+            "one_core": {"value": round(one, 3), "cores": 1,
+                         "sample": f"{slab_steps} steps of a {n}x{n}x{nz_slab} slab, {dt_one:.1f} s"}}
+
+
+def bench_ensemble(steps: int = 500, count: int = 64) -> dict:
+    """The batched-ensemble workload (capaz_amd.workloads.ensemble: 64 two-phase
+    realizations, edges drawn from {32, ..., 96}): `steps` steps of every
+    realization as one graph launch of the batch kernels, with the launch
+    shape from the autotuner, against the same steps run realization after
+    realization with single-domain launches."""
+    from capaz_amd import capaz, hip, workloads
+
+    keys, reals = workloads.ensemble(count=count)
+    cells = sum(int(np.prod(r["dims"])) for r in reals)
+    b = capaz.Batch(keys, reals)
+    tune_ms = b.autotune(4)
+    b.run(steps)  # warm-up, and the graph of `steps` steps is captured here
+    b.sync()
+    ev0, ev1 = hip.Event(), hip.Event()
     t0 = time.perf_counter()
-    d.run(steps)
-    dt = time.perf_counter() - t0
-    cells = 256 * 256 * nz_slab
-    return {"value": cells * steps / dt / 1e6, "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} steps of a 256x256x{nz_slab} slab of the headline workload "
-                      f"(same physics and seeded fields), oracle/liboracle.so on 1 core, {dt:.1f} s"}
+    ev0.record(b.stream())
+    b.run(steps)
+    ev1.record(b.stream())
+    b.sync()
+    wall_b = time.perf_counter() - t0
+    gpu_b = ev0.elapsed_ms(ev1) / 1e3
+    tuning = b.tuning()
+    # sequential: one realization at a time, single-domain fused launches
+    for d in b.domains:
+        d.run(2)
+    b.sync()
+    t0 = time.perf_counter()
+    for d in b.domains:
+        d.run(steps)
+        d.sync()
+    wall_s = time.perf_counter() - t0
+    b.close()
+    return {"workload": f"{count} two-phase realizations, edges from {{32,48,64,80,96}}, {steps} steps each "
+                        f"(lognormal correlated K, ln-sigma 1.0)",
+            "cells": cells, "steps": steps,
+            "batched": {"value": round(cells * steps / wall_b / 1e6, 1), "unit": "Mcell-steps/s",
+                        "ms_per_step": round(wall_b * 1e3 / steps, 4), "gpu_ms_per_step": round(gpu_b * 1e3 / steps, 4),
+                        "launch": "one HIP graph of all steps", "tuning": tuning,
+                        "autotune_ms_per_step": round(tune_ms, 4)},
+            "sequential": {"value": round(cells * steps / wall_s / 1e6, 1), "unit": "Mcell-steps/s",
+                           "ms_per_step": round(wall_s * 1e3 / steps, 4),
+                           "launch": "capaz_step per realization, one after another"},
+            "speedup": round(wall_s / wall_b, 2)}
 
 
 def main() -> int:
@@ -56,6 +138,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a copy of known size, to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -84,9 +167,12 @@ def main() -> int:
     dom.run(args.warmup, first_step=1)
     dom.sync()
 
-    # HIP events on the context's stream, where the fused-step kernels run
+    # HIP events on the context's stream, where the fused-step kernels run:
+    # one pair around the whole timed region and, from the library, one pair
+    # around every launch of the step's main kernel.
     stream = dom.stream()
     ev0, ev1 = hip.Event(), hip.Event()
+    dom.tune(timing=1)
     dom.sync()
     t0 = time.perf_counter()
     ev0.record(stream)
@@ -95,14 +181,27 @@ def main() -> int:
     dom.sync()
     wall = time.perf_counter() - t0
     gpu_ms = ev0.elapsed_ms(ev1)
-    print(f"bench: {args.steps} steps in {wall:.3f} s (events {gpu_ms:.1f} ms)", file=sys.stderr, flush=True)
+    kern_ms, launches = dom.kernel_time()
+    dom.tune(timing=0)
+    print(f"bench: {args.steps} steps in {wall:.3f} s (events {gpu_ms:.1f} ms, main kernel "
+          f"{kern_ms:.1f} ms over {launches} launches)", file=sys.stderr, flush=True)
 
-    launch_s = gpu_ms / 1e3 / args.steps  # one fused-step kernel per step
-    achieved = ALGO_BYTES_PER_CELL * cells / launch_s / 1e9
+    # The main kernel updates the interior cells; the boundary faces are a
+    # separate small kernel of the same step.
+    interior = (args.n - 2) ** 3
+    launch_s = kern_ms / 1e3 / launches
+    achieved = ALGO_BYTES_PER_CELL * interior / launch_s / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_fused_step", "algorithmic_bytes_per_cell": ALGO_BYTES_PER_CELL,
-            "avg_launch_us": round(launch_s * 1e6, 2)}
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": round(TRAFFIC_BYTES_PER_CELL * interior),
+            "kernel": "k_fused_z<3,2,64,8,true>", "algorithmic_bytes_per_cell": ALGO_BYTES_PER_CELL,
+            "cells_per_launch": interior, "avg_launch_us": round(launch_s * 1e6, 2),
+            "step_us": round(gpu_ms * 1e3 / args.steps, 2)}
+    fp64 = None
+    if FP64_FLOPS_PER_CELL:
+        flops = FP64_FLOPS_PER_CELL * interior / launch_s / 1e12
+        fp64 = {"bound": "fp64-valu", "achieved": round(flops, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(flops / FP64_PEAK_TFLOPS, 4), "flops_per_cell": FP64_FLOPS_PER_CELL}
 
     out = {
         "metric": METRIC,
@@ -118,13 +217,17 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"three-phase water/NAPL/gas {args.n}^3 FP64, Corey-2 relperms, lognormal K "
-                               f"(ln-sigma 0.5), 4 Newton iterations, top-face injection (config-1 physics)",
+                               f"(ln-sigma 0.5), 4 Newton iterations, top-face injection of water and gas",
                    "cells": cells, "grid": [args.n] * 3, "dt": keys["DT"], "newton_iterations": 4},
         "roofline": roof,
+        "fp64_roofline": fp64,
     }
+    if not args.no_ensemble:
+        print("bench: batched ensemble", file=sys.stderr, flush=True)
+        out["ensemble"] = bench_ensemble()
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
-        cb = cpu_baseline()
+        cb = cpu_baseline(args.n)
         cb["gpu_over_cpu"] = round(out["value"] / cb["value"], 1)
         out["cpu_baseline"] = cb
     dom.close()

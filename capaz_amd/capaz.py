@@ -51,10 +51,22 @@ EXPORTS = {
     "capaz_step": (ctypes.c_int, ["ctx", ctypes.c_long, ctypes.c_long]),
     "capaz_tune": (ctypes.c_int, ["ctx", ctypes.c_char_p, ctypes.c_int]),
     "capaz_get_tuning": (ctypes.c_int, ["ctx", ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
+    "capaz_kernel_time": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]),
     "capaz_sync": (ctypes.c_int, ["ctx"]),
     "capaz_stream": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_void_p)]),
     "capaz_load_face": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "capaz_save_face": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "capaz_batch_create": (ctypes.c_int, ["params_p", ctypes.c_int, ctypes.POINTER(ctypes.c_int), "ctx_pp"]),
+    "capaz_batch_destroy": (ctypes.c_int, ["ctx"]),
+    "capaz_batch_last_error": (ctypes.c_char_p, ["ctx"]),
+    "capaz_batch_size": (ctypes.c_int, ["ctx"]),
+    "capaz_batch_domain": (ctypes.c_int, ["ctx", ctypes.c_int, "ctx_pp"]),
+    "capaz_batch_step": (ctypes.c_int, ["ctx", ctypes.c_long, ctypes.c_long]),
+    "capaz_batch_tune": (ctypes.c_int, ["ctx", ctypes.c_char_p, ctypes.c_int]),
+    "capaz_batch_get_tuning": (ctypes.c_int, ["ctx", ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
+    "capaz_batch_autotune": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "capaz_batch_sync": (ctypes.c_int, ["ctx"]),
+    "capaz_batch_stream": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_void_p)]),
 }
 
 
@@ -139,16 +151,21 @@ def params_from_keys(keys: dict, device: int = 0) -> Params:
 class Domain:
     """One single-GPU domain: a capaz_ctx and every field of its model."""
 
-    def __init__(self, keys: dict, fields: dict[str, np.ndarray] | None = None, device: int = 0):
+    def __init__(self, keys: dict, fields: dict[str, np.ndarray] | None = None, device: int = 0,
+                 _ctx: ctypes.c_void_p | None = None):
         L = load()
         self.keys = dict(keys)
         self.params = params_from_keys(keys, device)
         self.shape = (self.params.Nz, self.params.Ny, self.params.Nx)
         self.n = int(np.prod(self.shape))
-        ctx = ctypes.c_void_p()
-        rc = L.capaz_create(ctypes.byref(self.params), ctypes.byref(ctx))
-        if rc != 0:
-            raise CapazError(f"capaz_create failed ({rc}): {L.capaz_last_error(None).decode()}")
+        self._owned = _ctx is None
+        if _ctx is None:
+            ctx = ctypes.c_void_p()
+            rc = L.capaz_create(ctypes.byref(self.params), ctypes.byref(ctx))
+            if rc != 0:
+                raise CapazError(f"capaz_create failed ({rc}): {L.capaz_last_error(None).decode()}")
+        else:
+            ctx = _ctx  # a batch member: the batch owns the context
         self._ctx = ctx
         self.names = [f for f in FIELDS if L.capaz_has_field(ctx, FIELD_ID[f])]
         if fields:
@@ -227,25 +244,116 @@ class Domain:
         self._check(load().capaz_step(self._ctx, first_step, nsteps), "capaz_step")
 
     def tune(self, **kw) -> None:
-        """Launch shape of the fused step (capaz_tune): fused_kernel, tile_y, kchunk."""
+        """Launch shape of the fused step (capaz_tune): fused_kernel, tile_y,
+        kchunk; timing=1 records events around the main kernel of each step."""
         for k, v in kw.items():
             self._check(load().capaz_tune(self._ctx, k.encode(), int(v)), f"capaz_tune({k}={v})")
 
     def tuning(self) -> dict:
         out = {}
-        for k in ("fused_kernel", "tile_y", "kchunk"):
+        for k in ("fused_kernel", "tile_y", "kchunk", "timing"):
             v = ctypes.c_int()
             self._check(load().capaz_get_tuning(self._ctx, k.encode(), ctypes.byref(v)), "capaz_get_tuning")
             out[k] = v.value
         return out
+
+    def kernel_time(self) -> tuple[float, int]:
+        """(total ms, launches) of the fused step's main kernel since timing was set."""
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        self._check(load().capaz_kernel_time(self._ctx, ctypes.byref(ms), ctypes.byref(n)), "capaz_kernel_time")
+        return ms.value, n.value
 
     def sync(self) -> None:
         self._check(load().capaz_sync(self._ctx), "capaz_sync")
 
     def close(self) -> None:
         if getattr(self, "_ctx", None):
-            load().capaz_destroy(self._ctx)
+            if self._owned:
+                load().capaz_destroy(self._ctx)
             self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """An ensemble of realizations advanced together (capaz_batch): the
+    physics of `keys`, one member per dict of `members` with its grid under
+    "dims" = (Nx, Ny, Nz) and its initial fields under their names.
+    ``domains[r]`` is realization r as a Domain (owned by the batch)."""
+
+    def __init__(self, keys: dict, members: list[dict], device: int = 0):
+        L = load()
+        if not members:
+            raise CapazError("a batch needs at least one realization")
+        dims = [int(d) for m in members for d in m["dims"]]
+        nx, ny, nz = dims[:3]
+        self.keys = dict(keys)
+        self.params = params_from_keys(dict(keys, NX=nx, NY=ny, NZ=nz), device)
+        arr = (ctypes.c_int * len(dims))(*dims)
+        b = ctypes.c_void_p()
+        rc = L.capaz_batch_create(ctypes.byref(self.params), len(members), arr, ctypes.byref(b))
+        if rc != 0:
+            raise CapazError(f"capaz_batch_create failed ({rc}): {L.capaz_batch_last_error(None).decode()}")
+        self._b = b
+        self.domains = []
+        for r, m in enumerate(members):
+            c = ctypes.c_void_p()
+            self._check(L.capaz_batch_domain(b, r, ctypes.byref(c)), "capaz_batch_domain")
+            x, y, z = m["dims"]
+            d = Domain(dict(keys, NX=x, NY=y, NZ=z), None, device, _ctx=c)
+            for name, v in m.items():
+                if name != "dims":
+                    d.set_field(name, v)
+            self.domains.append(d)
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != 0:
+            raise CapazError(f"{what} failed ({rc}): {load().capaz_batch_last_error(self._b).decode()}")
+
+    def __len__(self) -> int:
+        return len(self.domains)
+
+    def run(self, nsteps: int, first_step: int = 1) -> None:
+        """capaz_batch_step: nsteps fused steps of every realization."""
+        self._check(load().capaz_batch_step(self._b, first_step, nsteps), "capaz_batch_step")
+
+    def tune(self, **kw) -> None:
+        """capaz_batch_tune: tile_x, tile_y, kchunk, persistent, graph."""
+        for k, v in kw.items():
+            self._check(load().capaz_batch_tune(self._b, k.encode(), int(v)), f"capaz_batch_tune({k}={v})")
+
+    def tuning(self) -> dict:
+        out = {}
+        for k in ("tile_x", "tile_y", "kchunk", "persistent", "graph"):
+            v = ctypes.c_int()
+            self._check(load().capaz_batch_get_tuning(self._b, k.encode(), ctypes.byref(v)), "capaz_batch_get_tuning")
+            out[k] = v.value
+        return out
+
+    def autotune(self, steps: int = 4) -> float:
+        """capaz_batch_autotune: keeps the fastest launch shape; returns its ms per step."""
+        ms = ctypes.c_double()
+        self._check(load().capaz_batch_autotune(self._b, steps, ctypes.byref(ms)), "capaz_batch_autotune")
+        return ms.value
+
+    def stream(self) -> int:
+        s = ctypes.c_void_p()
+        self._check(load().capaz_batch_stream(self._b, ctypes.byref(s)), "capaz_batch_stream")
+        return s.value or 0
+
+    def sync(self) -> None:
+        self._check(load().capaz_batch_sync(self._b), "capaz_batch_sync")
+
+    def close(self) -> None:
+        if getattr(self, "_b", None):
+            for d in self.domains:
+                d.close()
+            load().capaz_batch_destroy(self._b)
+            self._b = None
 
     def __del__(self):
         try:

@@ -18,32 +18,18 @@
 #include <string>
 #include <utility>
 
-#include "../../include/capaz.h"
-#include "capaz_derive.h"
-#include "capaz_launch.h"
+#include "capaz_ctx.h"
 
 using capaz::DevFields;
 using capaz::DevParams;
 
-struct capaz_ctx {
-    capaz_params user;
-    DevParams p;
-    hipStream_t stream = nullptr;
-    double* cur[capaz::F_COUNT] = {};
-    double* alt[capaz::F_COUNT] = {};
-    bool primary[capaz::F_COUNT] = {};
-    bool stale = false;     // intermediates belong to an unrun materialisation
-    bool m_checked = false; // m has been scanned for zeros since it last changed
-    bool m_has_zero = false;
-    int* d_flag = nullptr;
-    double* d_dtm = nullptr;  // dt / m per cell, refreshed with the porosity scan
-    capaz::Tuning tuning;
-    std::string err;
-};
+
 
 namespace {
-
 thread_local std::string g_create_error;
+}  // namespace
+
+namespace capaz::detail {
 
 int fail(capaz_ctx* c, int code, const std::string& msg) {
     if (c)
@@ -57,6 +43,13 @@ int hip_fail(capaz_ctx* c, hipError_t e, const char* what) {
     return fail(c, e == hipErrorOutOfMemory ? CAPAZ_ERR_NOMEM : CAPAZ_ERR_DEVICE,
                 std::string(what) + ": " + hipGetErrorString(e));
 }
+
+}  // namespace capaz::detail
+
+namespace {
+
+using capaz::detail::fail;
+using capaz::detail::hip_fail;
 
 #define CAPAZ_HIP(c, call)                                   \
     do {                                                     \
@@ -124,6 +117,39 @@ int check_field(capaz_ctx* c, int field, size_t n) {
     return CAPAZ_OK;
 }
 
+constexpr size_t kEventPool = 512;
+
+// Waits for the recorded event pairs and adds their elapsed times to the totals.
+int drain_events(capaz_ctx* c) {
+    if (!c->ev_used) return CAPAZ_OK;
+    CAPAZ_HIP(c, hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < c->ev_used; ++i) {
+        float ms = 0.f;
+        CAPAZ_HIP(c, hipEventElapsedTime(&ms, c->ev[i].begin, c->ev[i].end));
+        c->timed_ms += ms;
+    }
+    c->timed_launches += (long)c->ev_used;
+    c->ev_used = 0;
+    return CAPAZ_OK;
+}
+
+// The next free event pair of the pool, or NULL when timing is off.
+int next_events(capaz_ctx* c, const capaz::KernelEvents** out) {
+    *out = nullptr;
+    if (!c->timing) return CAPAZ_OK;
+    if (c->ev.empty()) {
+        c->ev.resize(kEventPool, capaz::KernelEvents{nullptr, nullptr});
+        for (auto& e : c->ev) {
+            CAPAZ_HIP(c, hipEventCreate(&e.begin));
+            CAPAZ_HIP(c, hipEventCreate(&e.end));
+        }
+    }
+    int rc;
+    if (c->ev_used == c->ev.size() && (rc = drain_events(c))) return rc;
+    *out = &c->ev[c->ev_used++];
+    return CAPAZ_OK;
+}
+
 int scan_m(capaz_ctx* c) {
     if (c->m_checked) return CAPAZ_OK;
     CAPAZ_HIP(c, hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
@@ -138,6 +164,12 @@ int scan_m(capaz_ctx* c) {
 }
 
 }  // namespace
+
+namespace capaz::detail {
+int materialize(capaz_ctx* c) { return ::materialize(c); }
+int scan_m(capaz_ctx* c) { return ::scan_m(c); }
+int check_ctx(capaz_ctx* c) { return ::check_ctx(c); }
+}  // namespace capaz::detail
 
 extern "C" {
 
@@ -200,6 +232,10 @@ int capaz_destroy(capaz_ctx* c) {
     }
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->d_dtm) (void)hipFree(c->d_dtm);
+    for (auto& e : c->ev) {
+        if (e.begin) (void)hipEventDestroy(e.begin);
+        if (e.end) (void)hipEventDestroy(e.end);
+    }
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return CAPAZ_OK;
@@ -323,7 +359,9 @@ int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
         io.Sn1 = c->alt[F_S_n];
         io.Pn1 = c->alt[F_P_n];
         io.Pg1 = c->alt[F_P_g];
-        CAPAZ_HIP(c, launch_fused(c->p, io, c->tuning, c->stream));
+        const KernelEvents* ev;
+        if ((rc = next_events(c, &ev))) return rc;
+        CAPAZ_HIP(c, launch_fused(c->p, io, c->tuning, c->stream, ev));
         for (int f = 0; f < F_COUNT; ++f)
             if (c->primary[f]) std::swap(c->cur[f], c->alt[f]);
         c->stale = true;
@@ -340,7 +378,13 @@ int capaz_tune(capaz_ctx* c, const char* key, int value) {
         c->tuning.tile_y = value;
     else if (k == "kchunk" && value >= 1 && value <= 1 << 20)
         c->tuning.kchunk = value;
-    else
+    else if (k == "timing" && (value == 0 || value == 1)) {
+        int rc;
+        if ((rc = check_ctx(c)) || (rc = drain_events(c))) return rc;
+        c->timing = value == 1;
+        c->timed_ms = 0.;
+        c->timed_launches = 0;
+    } else
         return fail(c, CAPAZ_ERR_INVALID, "capaz_tune: unknown key or value out of range");
     return CAPAZ_OK;
 }
@@ -351,7 +395,18 @@ int capaz_get_tuning(const capaz_ctx* c, const char* key, int* value) {
     if (k == "fused_kernel") *value = c->tuning.fused_kernel;
     else if (k == "tile_y") *value = c->tuning.tile_y;
     else if (k == "kchunk") *value = c->tuning.kchunk;
+    else if (k == "timing") *value = c->timing ? 1 : 0;
     else return CAPAZ_ERR_INVALID;
+    return CAPAZ_OK;
+}
+
+int capaz_kernel_time(capaz_ctx* c, double* total_ms, long* launches) {
+    int rc;
+    if ((rc = check_ctx(c))) return rc;
+    if (!total_ms || !launches) return fail(c, CAPAZ_ERR_INVALID, "NULL argument");
+    if ((rc = drain_events(c))) return rc;
+    *total_ms = c->timed_ms;
+    *launches = c->timed_launches;
     return CAPAZ_OK;
 }
 

@@ -1,8 +1,8 @@
 // The fused time-step kernel for gfx950: one pass over the primary state per
 // time step (two- and three-phase models without the energy equation).
 //
-// Mapping.  A workgroup owns a 64 x TY block of interior columns (x fastest,
-// so each wavefront row streams 64 consecutive doubles of every field) and a
+// Mapping.  A workgroup owns a TX x TY block of interior columns (x fastest,
+// so each wavefront row streams TX consecutive doubles of every field) and a
 // chunk of planes along z, which it walks plane by plane.  For the current
 // plane the closure coefficients of each cell -- phase pressures, Xi and
 // densities, the values the upwind flux reads from a neighbour -- are computed
@@ -138,26 +138,31 @@ __device__ __forceinline__ void write_boundary(const DevParams& p, const StepIO&
     }
 }
 
-template <int NV, int TY>
+template <int NV, int TX, int TY>
 struct FusedLds {
-    double v[NV][TY + 2][66];
+    double v[NV][TY + 2][TX + 2];
 };
 
+// One tile of the fused step: interior columns i0 .. i0+TX-1 (i0 = first
+// interior i + TX*bx), j0 .. j0+TY-1 (j0 = 1 + TY*by), planes k0 .. k0+kchunk-1
+// (k0 = first interior k + kchunk*bz), on a TX x TY workgroup (TY >= 3,
+// TX >= 2*TY: rows 0-2 of the workgroup also load the halo ring).
+//
 // CARRY: keep the next plane's coefficients of the own column in registers
 // (computed for the z flux) as the next centre, or recompute the centre at
 // the top of every plane (fewer live registers across the Newton solve).
-template <int MODEL, int CLOSURE, int TY, bool CARRY>
-__global__ void __launch_bounds__(64 * TY) k_fused_z(DevParams p, StepIO io, int kchunk) {
+template <int MODEL, int CLOSURE, int TX, int TY, bool CARRY>
+__device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& io, int kchunk, int bx, int by, int bz,
+                                             FusedLds<3 * MODEL, TX, TY>& lds) {
+    static_assert(TY >= 3 && TX >= 2 * TY, "halo ring needs TY >= 3 and TX >= 2*TY");
     constexpr int NPH = MODEL;
-    constexpr int NV = 3 * NPH;
-    __shared__ FusedLds<NV, TY> lds;
 
     const int tx = threadIdx.x, ty = threadIdx.y;
     const bool hasx = p.Nx >= 3, hasz = p.Nz >= 3;
     const int ilo = hasx ? 1 : 0, ihi = hasx ? p.Nx - 2 : 0;
     const int klo = hasz ? 1 : 0, khi = hasz ? p.Nz - 2 : 0;
-    const int i0 = ilo + 64 * (int)blockIdx.x, j0 = 1 + TY * (int)blockIdx.y;
-    const int k0 = klo + kchunk * (int)blockIdx.z;
+    const int i0 = ilo + TX * bx, j0 = 1 + TY * by;
+    const int k0 = klo + kchunk * bz;
     const int k1 = min(khi, k0 + kchunk - 1);
     const int i = i0 + tx, j = j0 + ty;
     const bool own_in_grid = i < p.Nx && j < p.Ny;
@@ -169,7 +174,7 @@ __global__ void __launch_bounds__(64 * TY) k_fused_z(DevParams p, StepIO io, int
     if (ty == 0) { hi = i; hj = j0 - 1; hx = tx + 1; hy = 0; }
     else if (ty == 1) { hi = i; hj = j0 + TY; hx = tx + 1; hy = TY + 1; }
     else if (ty == 2 && tx < TY) { hi = i0 - 1; hj = j0 + tx; hx = 0; hy = tx + 1; }
-    else if (ty == 2 && tx >= TY && tx < 2 * TY) { hi = i0 + 64; hj = j0 + tx - TY; hx = 65; hy = tx - TY + 1; }
+    else if (ty == 2 && tx >= TY && tx < 2 * TY) { hi = i0 + TX; hj = j0 + tx - TY; hx = TX + 1; hy = tx - TY + 1; }
     const bool halo_in_grid = hi >= 0 && hi < p.Nx && hj >= 0 && hj < p.Ny && (hasx || hi == i0 + tx);
 
     const int64_t col = (int64_t)i + (int64_t)j * p.sy;
@@ -270,16 +275,22 @@ __global__ void __launch_bounds__(64 * TY) k_fused_z(DevParams p, StepIO io, int
     }
 }
 
-// Boundary cells after k_fused_z: thread t enumerates the j faces (all i, k),
-// then the x faces (j interior, all k), then the z faces (i, j interior).
+template <int MODEL, int CLOSURE, int TX, int TY, bool CARRY>
+__global__ void __launch_bounds__(TX * TY) k_fused_z(DevParams p, StepIO io, int kchunk) {
+    __shared__ FusedLds<3 * MODEL, TX, TY> lds;
+    fused_z_tile<MODEL, CLOSURE, TX, TY, CARRY>(p, io, kchunk, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, lds);
+}
+
+// Boundary cell number t of the step after the interior tiles (t enumerates
+// the j faces (all i, k), then the x faces (j interior, all k), then the z
+// faces (i, j interior)); nothing for t past the last face cell.
 template <int MODEL, int CLOSURE>
-__global__ void __launch_bounds__(256) k_fused_faces(DevParams p, StepIO io) {
+__device__ __forceinline__ void fused_face(const DevParams& p, const StepIO& io, int64_t t) {
     const bool hasx = p.Nx >= 3, hasz = p.Nz >= 3;
     const int64_t nj = 2LL * p.Nx * p.Nz;
     const int64_t nx = hasx ? 2LL * (p.Ny - 2) * p.Nz : 0;
     const int ni = hasx ? p.Nx - 2 : p.Nx;
     const int64_t nzf = hasz ? 2LL * ni * (p.Ny - 2) : 0;
-    int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
     int i, j, k;
     if (t < nj) {
         j = (t & 1) ? p.Ny - 1 : 0;
@@ -310,6 +321,11 @@ __global__ void __launch_bounds__(256) k_fused_faces(DevParams p, StepIO io) {
         ro_eff = (c1.ro[1] * Sn + c1.ro[0] * (1 - Sn)) * io.m[l1] * p.g * p.hy;
     }
     write_boundary<MODEL>(p, io, i, j, k, Pw, Sw, Sn, ro_eff);
+}
+
+template <int MODEL, int CLOSURE>
+__global__ void __launch_bounds__(256) k_fused_faces(DevParams p, StepIO io) {
+    fused_face<MODEL, CLOSURE>(p, io, (int64_t)blockIdx.x * 256 + threadIdx.x);
 }
 
 inline int64_t fused_face_cells(const DevParams& p) {

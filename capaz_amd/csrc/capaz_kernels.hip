@@ -124,39 +124,46 @@ hipError_t launch_fused_step(const DevParams& p, const StepIO& io, hipStream_t s
 }
 
 template <int MODEL, int CLOSURE, int TY, bool CARRY>
-hipError_t launch_z(const DevParams& p, const StepIO& io, int kchunk, hipStream_t s) {
+hipError_t launch_z(const DevParams& p, const StepIO& io, int kchunk, hipStream_t s, const KernelEvents* ev) {
     const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
     if (kchunk < 1) kchunk = 1;
     dim3 grid((nx + 63) / 64, (ny + TY - 1) / TY, (nz + kchunk - 1) / kchunk);
-    hipLaunchKernelGGL((k_fused_z<MODEL, CLOSURE, TY, CARRY>), grid, dim3(64, TY), 0, s, p, io, kchunk);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    hipError_t e;
+    if (ev && (e = hipEventRecord(ev->begin, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL((k_fused_z<MODEL, CLOSURE, 64, TY, CARRY>), grid, dim3(64, TY), 0, s, p, io, kchunk);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (ev && (e = hipEventRecord(ev->end, s)) != hipSuccess) return e;
     hipLaunchKernelGGL((k_fused_faces<MODEL, CLOSURE>), grid_for(fused_face_cells(p)), dim3(256), 0, s, p, io);
     return hipGetLastError();
 }
 
 template <int MODEL, int CLOSURE>
-hipError_t launch_z_ty(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s) {
+hipError_t launch_z_ty(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s, const KernelEvents* ev) {
     const bool carry = t.fused_kernel == 1;
     if (t.tile_y == 8)
-        return carry ? launch_z<MODEL, CLOSURE, 8, true>(p, io, t.kchunk, s)
-                     : launch_z<MODEL, CLOSURE, 8, false>(p, io, t.kchunk, s);
-    return carry ? launch_z<MODEL, CLOSURE, 4, true>(p, io, t.kchunk, s)
-                 : launch_z<MODEL, CLOSURE, 4, false>(p, io, t.kchunk, s);
+        return carry ? launch_z<MODEL, CLOSURE, 8, true>(p, io, t.kchunk, s, ev)
+                     : launch_z<MODEL, CLOSURE, 8, false>(p, io, t.kchunk, s, ev);
+    return carry ? launch_z<MODEL, CLOSURE, 4, true>(p, io, t.kchunk, s, ev)
+                 : launch_z<MODEL, CLOSURE, 4, false>(p, io, t.kchunk, s, ev);
 }
 
 // CLOSURE ids of the fused kernels: 0 reference, 1 Corey with a run-time
 // exponent, 2 Corey with exponent 2 (no pow in the kernel).
 template <int MODEL>
-hipError_t launch_z_model(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s) {
-    if (p.closure == 0) return launch_z_ty<MODEL, 0>(p, io, t, s);
-    if (p.corey_n == 2.) return launch_z_ty<MODEL, 2>(p, io, t, s);
-    return launch_z_ty<MODEL, 1>(p, io, t, s);
+hipError_t launch_z_model(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s, const KernelEvents* ev) {
+    if (p.closure == 0) return launch_z_ty<MODEL, 0>(p, io, t, s, ev);
+    if (p.corey_n == 2.) return launch_z_ty<MODEL, 2>(p, io, t, s, ev);
+    return launch_z_ty<MODEL, 1>(p, io, t, s, ev);
 }
 
-hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s) {
-    if (t.fused_kernel == 0) return launch_fused_step(p, io, s);
-    return p.model == 3 ? launch_z_model<3>(p, io, t, s) : launch_z_model<2>(p, io, t, s);
+hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s, const KernelEvents* ev) {
+    if (t.fused_kernel == 0) {
+        hipError_t e;
+        if (ev && (e = hipEventRecord(ev->begin, s)) != hipSuccess) return e;
+        if ((e = launch_fused_step(p, io, s)) != hipSuccess) return e;
+        return ev ? hipEventRecord(ev->end, s) : hipSuccess;
+    }
+    return p.model == 3 ? launch_z_model<3>(p, io, t, s, ev) : launch_z_model<2>(p, io, t, s, ev);
 }
 
 hipError_t launch_dtm(const DevParams& p, const double* m, double* dtm, hipStream_t s) {

@@ -19,9 +19,17 @@ struct Tuning {
     int kchunk = 16;
 };
 
+// Optional events recorded around the fused step's main kernel (the
+// z-marching tile kernel, or the one-thread-per-cell kernel), not its
+// boundary-face kernel.
+struct KernelEvents {
+    hipEvent_t begin, end;
+};
+
 hipError_t launch_entry(Entry e, const DevParams& p, const DevFields& F, hipStream_t s);
 hipError_t launch_fused_step(const DevParams& p, const StepIO& io, hipStream_t s);
-hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s);
+hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s,
+                        const KernelEvents* ev = nullptr);
 hipError_t launch_dtm(const DevParams& p, const double* m, double* dtm, hipStream_t s);
 hipError_t launch_copy_interior(const DevParams& p, const double* src, double* dst, hipStream_t s);
 hipError_t launch_m_zero(const DevParams& p, const double* m, int* flag, hipStream_t s);

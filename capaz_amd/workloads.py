@@ -1,18 +1,19 @@
-"""Seeded synthetic inputs of the benchmark configurations.
+"""Seeded synthetic inputs of the benchmark workloads.
 
 Each builder returns ``(keys, fields)``: a deck of defines.ini keys
 (main.cpp:1327-1626) plus the backend's MODEL / ENERGY / CLOSURE / COREY_N
 keys, and the initial primary fields as (Nz, Ny, Nx) float64 arrays, x
-fastest.  Random fields come from numpy's PCG64 with seed 1000 + config
-index.  "Top" is j = 0 (gravity along +j, cpu.cpp:392).
+fastest.  Random fields come from numpy's PCG64 (two-phase Buckley-Leverett
+seed 1000, three-phase injection seed 1001).  "Top" is j = 0 (gravity along
++j, cpu.cpp:392).
 
 Parameter substitutions (the decks as specified are unstable for this
 explicit scheme; the stable values used are):
-  * three-phase (config 1 and the 256^3 headline): dt = 1e-5 s instead of
+  * three-phase (128^3 and the 256^3 headline): dt = 1e-5 s instead of
     1e-4 s.  At h = 0.01 m and beta = 4.5e-10 1/Pa the scheme diverges within
     20 steps at dt = 1e-4 (24^3, oracle and reference alike); the threshold
     measured at 24^3 is ~6e-5 s and lognormal K tails at 256^3 tighten it.
-  * two-phase (config 0): beta_w = beta_n = 1e-6 1/Pa instead of 4.5e-10.
+  * two-phase Buckley-Leverett (40^3): beta_w = beta_n = 1e-6 1/Pa instead of 4.5e-10.
     Without gas the only compressibility is beta, and the pressure-diffusion
     limit h^2*phi*beta*mu/(6*K*k) is ~2.5e-7 s at 4.5e-10, 400x below
     dt = 1e-4; at 1e-6 it is ~5.6e-4 s.
@@ -29,7 +30,7 @@ def _porosity_noise(rng, shape):
     return np.clip(0.4 * (1.0 + 0.05 * rng.standard_normal(shape)), 0.3, 0.5)
 
 
-def config0(n: int = 40):
+def buckley_leverett_2ph(n: int = 40):
     """Two-phase water/NAPL, n^3, Corey-2 (S_wr 0.1, S_nr 0.05), scalar K,
     S_w0 = 0.15, top face injects S_w = 0.9 at P = 1.2e5 Pa.
 
@@ -48,15 +49,15 @@ def config0(n: int = 40):
     return keys, fields
 
 
-def config1(n: int = 128, seed: int = 1001):
-    """Three-phase water/NAPL/gas on an n^3 grid (n = 128 in config 1; the
-    headline measures the same physics at n = 256): 4 Newton iterations,
+def three_phase_injection(n: int = 128, seed: int = 1001):
+    """Three-phase water/NAPL/gas on an n^3 grid (128^3 as specified; the
+    headline measures the same physics at 256^3): 4 Newton iterations,
     ideal-gas density with ro0_g = 1.2, mu_g = 1.8e-5, Corey-2 relperms
     (S_wr 0.1, S_nr 0.05, S_gr 0), S0 = (0.2, 0.7, 0.1), the top face injects
     (0.6, 0.1, 0.3), porosity 0.4(1 + 0.05 N(0,1)) clipped to [0.3, 0.5],
     K lognormal with geometric mean 6.64e-11 m^2 and ln-sigma 0.5 (white
     noise per cell).  h = 0.01 m, beta = 4.5e-10 1/Pa, mu_w 1e-3, mu_n 5e-3,
-    ro0 1000 / 850 as config 0; P = P_atm everywhere initially (the
+    ro0 1000 / 850 as in the two-phase workload; P = P_atm everywhere initially (the
     reference's three-phase data_initialization, three-phase.cpp:470-472)."""
     rng = np.random.default_rng(seed)
     shape = (n, n, n)
@@ -72,5 +73,46 @@ def config1(n: int = 128, seed: int = 1001):
 
 
 def headline(n: int = 256):
-    """The benchmark workload: config 1 physics at 256^3 (seed 1001)."""
-    return config1(n, seed=1001)
+    """The benchmark workload: the three-phase injection physics at 256^3 (seed 1001)."""
+    return three_phase_injection(n, seed=1001)
+
+
+def _correlated_lnK(rng, n: int, corr_cells: float, ln_sigma: float) -> np.ndarray:
+    """ln K on an n^3 grid: white noise smoothed by a Gaussian filter of
+    `corr_cells` cells standard deviation, rescaled to mean ln(K_MEAN) and
+    standard deviation `ln_sigma`."""
+    from scipy.ndimage import gaussian_filter
+
+    w = gaussian_filter(rng.standard_normal((n, n, n)), sigma=corr_cells, mode="reflect")
+    w = (w - w.mean()) / w.std()
+    return np.log(K_MEAN) + ln_sigma * w
+
+
+ENSEMBLE_EDGES = (32, 48, 64, 80, 96)
+
+
+def ensemble(count: int = 64, seed: int = 1003, dt: float = 1e-6):
+    """Ensemble of two-phase Buckley-Leverett realizations: each cube edge
+    drawn uniformly from {32, 48, 64, 80, 96}; K lognormal with geometric mean
+    6.64e-11 m^2 and ln-sigma 1.0, correlated by a Gaussian filter of 4 cells;
+    porosity 0.4 (K / K_mean)^0.1 clipped to [0.25, 0.5]; physics and
+    initial state of buckley_leverett_2ph, except dt = 1e-6 s: the K tails
+    (up to ~500 K_mean) put the explicit pressure-diffusion limit near 1e-6 s
+    (dt = 1e-5 diverges within 100 steps on a 96^3 realization, 1e-6 runs
+    500 steps on the realization of largest K).  Returns (keys, [fields, ...]):
+    keys of the common physics (grid keys are per realization, in each
+    fields dict under "dims" as (Nx, Ny, Nz))."""
+    rng = np.random.default_rng(seed)
+    edges = rng.choice(np.array(ENSEMBLE_EDGES), size=count)
+    keys, _ = buckley_leverett_2ph(int(edges[0]))
+    # SOURCE covers the whole top face of the largest realization
+    keys = dict(keys, DT=dt, SOURCE=max(ENSEMBLE_EDGES))
+    out = []
+    for n in edges:
+        n = int(n)
+        lnK = _correlated_lnK(rng, n, 4.0, 1.0)
+        K = np.exp(lnK)
+        m = np.clip(0.4 * (K / K_MEAN) ** 0.1, 0.25, 0.5)
+        shape = (n, n, n)
+        out.append(dict(dims=(n, n, n), P_w=np.full(shape, keys["P_ATM"]), S_n=np.full(shape, 0.85), m=m, K=K))
+    return keys, out

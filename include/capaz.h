@@ -162,9 +162,18 @@ int capaz_step(capaz_ctx* ctx, long first_step, long nsteps);
  *                   2: as 1, recomputing the centre coefficients per plane
  *   "tile_y"        4 or 8 (default 8)
  *   "kchunk"        planes per workgroup along z (default 16)
- * Returns CAPAZ_ERR_INVALID for an unknown key or value. */
+ *   "timing"        1: record HIP events around each fused step's main
+ *                   kernel (read with capaz_kernel_time); 0: off (default).
+ *                   Setting it resets the totals.
+ * Returns CAPAZ_ERR_INVALID for an unknown key or value.  No counterpart in
+ * the reference (gpu.cu's launch shape is fixed). */
 int capaz_tune(capaz_ctx* ctx, const char* key, int value);
 int capaz_get_tuning(const capaz_ctx* ctx, const char* key, int* value);
+
+/* Total device time (ms) and number of the fused step's main-kernel launches
+ * timed since "timing" was last set; waits for queued work.  No counterpart
+ * in the reference. */
+int capaz_kernel_time(capaz_ctx* ctx, double* total_ms, long* launches);
 
 /* Waits for all work queued on the context. */
 int capaz_sync(capaz_ctx* ctx);
@@ -179,6 +188,48 @@ int capaz_stream(capaz_ctx* ctx, void** stream);
  * plane's size; the plane is laid out as the reference's HostBuffer. */
 int capaz_load_face(capaz_ctx* ctx, int field, int axis, int side, double* dbuf);
 int capaz_save_face(capaz_ctx* ctx, int field, int axis, int side, const double* dbuf);
+
+/* ---- Batched ensembles ------------------------------------------------------
+ * No counterpart in the reference (one domain per process there): an
+ * ensemble of realizations of one physics, each with its own grid and fields,
+ * advanced together -- one launch of the interior kernel and one of the
+ * boundary kernel per step for all realizations, the n steps of a
+ * capaz_batch_step call captured as one HIP graph and replayed.  Results are
+ * those of capaz_step on each realization alone, bit for bit. */
+typedef struct capaz_batch capaz_batch;
+
+/* count realizations with the physics of `base` (its Nx/Ny/Nz ignored) and
+ * grids dims[3r], dims[3r+1], dims[3r+2] (Nx, Ny, Nz of realization r).  No
+ * energy model.  On failure *out is NULL and capaz_batch_last_error(NULL)
+ * says why. */
+int capaz_batch_create(const capaz_params* base, int count, const int* dims, capaz_batch** out);
+/* Frees the batch and its realizations' contexts. */
+int capaz_batch_destroy(capaz_batch* b);
+const char* capaz_batch_last_error(const capaz_batch* b);
+int capaz_batch_size(const capaz_batch* b);
+/* The context of realization r (owned by the batch: do not destroy it).  All
+ * single-domain calls work on it -- set/get fields, the entry points,
+ * capaz_step -- and see the batch's steps. */
+int capaz_batch_domain(capaz_batch* b, int r, capaz_ctx** ctx);
+/* nsteps fused steps of every realization (step numbers from first_step, as
+ * capaz_step).  Fails with CAPAZ_ERR_STATE if a realization has m == 0 cells. */
+int capaz_batch_step(capaz_batch* b, long first_step, long nsteps);
+/* Launch shape (results do not change):
+ *   "tile_x", "tile_y"  interior tile: 64x8 (default), 64x4 or 32x8 columns
+ *   "kchunk"            planes per tile (default 16)
+ *   "persistent"        1: a resident grid taking tiles from a queue
+ *                       (default); 0: one workgroup per tile
+ *   "graph"             1: capture each run of steps as a HIP graph (default) */
+int capaz_batch_tune(capaz_batch* b, const char* key, int value);
+int capaz_batch_get_tuning(const capaz_batch* b, const char* key, int* value);
+/* Times every tile shape x kchunk in {8,16,32} x persistent in {1,0} over
+ * `steps` steps of this ensemble and keeps the fastest; the realizations'
+ * state is restored afterwards.  *best_ms (may be NULL): its time per step. */
+int capaz_batch_autotune(capaz_batch* b, int steps, double* best_ms);
+/* Waits for the batch's and its realizations' queued work. */
+int capaz_batch_sync(capaz_batch* b);
+/* The batch's hipStream_t, as void*. */
+int capaz_batch_stream(capaz_batch* b, void** stream);
 
 #ifdef __cplusplus
 }

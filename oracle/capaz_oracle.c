@@ -23,6 +23,21 @@
 #include <stddef.h>
 #include <stdlib.h>
 
+/* Built with -fopenmp, the per-cell passes split their outer loop over
+ * threads; the cells are independent (see above), so results do not depend
+ * on the thread count. */
+#ifdef _OPENMP
+#include <omp.h>
+#define OMP_PRAGMA(x) _Pragma(#x)
+#define OMP_CELLS OMP_PRAGMA(omp parallel for schedule(static))
+int oracle_threads(void) { return omp_get_max_threads(); }
+void oracle_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+#else
+#define OMP_CELLS
+int oracle_threads(void) { return 1; }
+void oracle_set_threads(int n) { (void)n; }
+#endif
+
 #define IDX(p, i, j, k) ((size_t)(i) + (size_t)(j) * (size_t)(p)->Nx + (size_t)(k) * (size_t)(p)->Nx * (size_t)(p)->Ny)
 
 /* ---- cell predicates ------------------------------------------------------ */
@@ -188,6 +203,7 @@ static double d_ro_e(const oracle_params* p, double P, double T, char ph, char v
 
 void oracle_S_calculation(const oracle_params* p, oracle_fields* f) {
     size_t n = (size_t)p->Nx * p->Ny * p->Nz;
+    OMP_CELLS
     for (size_t l = 0; l < n; ++l) {
         if (p->model == 3)
             f->S_g[l] = 1. - f->S_w[l] - f->S_n[l];
@@ -272,6 +288,7 @@ static void assign_ro(const oracle_params* p, oracle_fields* f, size_t l) {
 }
 
 void oracle_ro_P_Xi_calculation(const oracle_params* p, oracle_fields* f) {
+    OMP_CELLS
     for (int k = 0; k < p->Nz; ++k)
         for (int j = 0; j < p->Ny; ++j)
             for (int i = 0; i < p->Nx; ++i) {
@@ -309,6 +326,7 @@ static void u_axis(const oracle_params* p, const double* Xi, const double* P, co
 
 void oracle_u_calculation(const oracle_params* p, oracle_fields* f) {
     size_t sx = 1, sy = (size_t)p->Nx, sz = (size_t)p->Nx * p->Ny;
+    OMP_CELLS
     for (int k = 0; k < p->Nz; ++k)
         for (int j = 0; j < p->Ny; ++j)
             for (int i = 0; i < p->Nx; ++i) {
@@ -352,6 +370,7 @@ static double phase_flux(const oracle_params* p, const double* P, const double* 
 void oracle_roS_calculation(const oracle_params* p, oracle_fields* f, double t) {
     (void)t;
     size_t sx = 1, sy = (size_t)p->Nx, sz = (size_t)p->Nx * p->Ny;
+    OMP_CELLS
     for (int k = 0; k < p->Nz; ++k)
         for (int j = 0; j < p->Ny; ++j)
             for (int i = 0; i < p->Nx; ++i) {
@@ -401,6 +420,7 @@ void oracle_roS_calculation(const oracle_params* p, oracle_fields* f, double t) 
 
 void oracle_H_E_current_calculation(const oracle_params* p, oracle_fields* f) {
     size_t n = (size_t)p->Nx * p->Ny * p->Nz;
+    OMP_CELLS
     for (size_t l = 0; l < n; ++l) {
         assign_H(p, f, l);
         f->E[l] = (f->m[l] * (f->S_w[l] * (f->ro_w[l] * f->H_w[l] - f->P_w[l]) +
@@ -441,6 +461,7 @@ static double E_flow_at(const oracle_params* p, const oracle_fields* f, size_t l
 }
 
 void oracle_E_calculation(const oracle_params* p, oracle_fields* f) {
+    OMP_CELLS
     for (int k = 0; k < p->Nz; ++k)
         for (int j = 0; j < p->Ny; ++j)
             for (int i = 0; i < p->Nx; ++i) {
@@ -600,6 +621,7 @@ static void newton_energy(const oracle_params* p, oracle_fields* f, size_t l) {
 }
 
 void oracle_P_S_calculation(const oracle_params* p, oracle_fields* f) {
+    OMP_CELLS
     for (int k = 0; k < p->Nz; ++k)
         for (int j = 0; j < p->Ny; ++j)
             for (int i = 0; i < p->Nx; ++i) {

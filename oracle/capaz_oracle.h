@@ -56,6 +56,10 @@ void oracle_time_step(const oracle_params* p, oracle_fields* f, double t);
 /* nsteps calls with t = s*dt, s = first_step .. first_step+nsteps-1. */
 void oracle_run(const oracle_params* p, oracle_fields* f, long first_step, long nsteps);
 
+/* Threads the per-cell passes use (1 unless built with OpenMP). */
+int oracle_threads(void);
+void oracle_set_threads(int n);
+
 /* gauss.cpp:6-92 and :94-104. */
 int oracle_reverse_matrix(double* a, int n);
 void oracle_mult_matrix_vector(double* result, const double* matr, const double* vect, int n);

@@ -71,6 +71,9 @@ def lib() -> ctypes.CDLL:
         L.oracle_run.argtypes = [P, F, ctypes.c_long, ctypes.c_long]
         L.oracle_reverse_matrix.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.oracle_relperm.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_double)]
+        L.oracle_threads.restype = ctypes.c_int
+        L.oracle_threads.argtypes = []
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
 
@@ -142,6 +145,15 @@ class OracleDomain:
 
     def run(self, nsteps: int, first_step: int = 1) -> None:
         lib().oracle_run(ctypes.byref(self.p), ctypes.byref(self._f), first_step, nsteps)
+
+
+def threads() -> int:
+    """Threads the oracle's per-cell passes use."""
+    return lib().oracle_threads()
+
+
+def set_threads(n: int) -> None:
+    lib().oracle_set_threads(n)
 
 
 def reverse_matrix(a: np.ndarray) -> tuple[int, np.ndarray]:

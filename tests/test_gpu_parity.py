@@ -118,7 +118,7 @@ def test_random_state_vs_oracle(model, dims, over, exact):
 def test_headline_physics_bitwise_at_96():
     """The benchmark physics (three-phase, Corey-2, lognormal K, top
     injection) at 96^3: one and three fused steps equal the oracle bit for bit."""
-    keys, st = workloads.config1(96, seed=7)
+    keys, st = workloads.three_phase_injection(96, seed=7)
     od = oracle.OracleDomain(keys, st)
     d = capaz.Domain(keys, st)
     od.run(1)
