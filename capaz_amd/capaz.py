@@ -56,6 +56,9 @@ EXPORTS = {
     "capaz_stream": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_void_p)]),
     "capaz_load_face": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "capaz_save_face": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "capaz_objective": (ctypes.c_int, ["ctx", ctypes.c_long, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "capaz_adjoint": (ctypes.c_int, ["ctx", ctypes.c_long, ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_double), "dbl_p", "dbl_p"]),
     "capaz_batch_create": (ctypes.c_int, ["params_p", ctypes.c_int, ctypes.POINTER(ctypes.c_int), "ctx_pp"]),
     "capaz_batch_destroy": (ctypes.c_int, ["ctx"]),
     "capaz_batch_last_error": (ctypes.c_char_p, ["ctx"]),
@@ -257,6 +260,24 @@ class Domain:
             out[k] = v.value
         return out
 
+    def objective(self, nsteps: int, obs: dict) -> float:
+        """capaz_objective: J after nsteps fused steps from the current state
+        (obs: cells, steps, S_w, S_n as in include/capaz.h)."""
+        o, keep = make_observations(obs["cells"], obs["steps"], obs["S_w"], obs["S_n"])
+        J = ctypes.c_double()
+        self._check(load().capaz_objective(self._ctx, nsteps, ctypes.byref(o), ctypes.byref(J)), "capaz_objective")
+        return J.value
+
+    def adjoint(self, nsteps: int, obs: dict, checkpoint_every: int = 0):
+        """capaz_adjoint: (J, dJ/d lnK, dJ/dm) after nsteps fused steps from the current state."""
+        o, keep = make_observations(obs["cells"], obs["steps"], obs["S_w"], obs["S_n"])
+        J = ctypes.c_double()
+        gK, gm = np.empty(self.shape), np.empty(self.shape)
+        self._check(load().capaz_adjoint(self._ctx, nsteps, ctypes.byref(o), checkpoint_every, ctypes.byref(J),
+                                         gK.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                         gm.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "capaz_adjoint")
+        return J.value, gK, gm
+
     def kernel_time(self) -> tuple[float, int]:
         """(total ms, launches) of the fused step's main kernel since timing was set."""
         ms, n = ctypes.c_double(), ctypes.c_long()
@@ -277,6 +298,27 @@ class Domain:
             self.close()
         except Exception:
             pass
+
+
+class Observations(ctypes.Structure):
+    """capaz_observations (include/capaz.h)."""
+    _fields_ = [("ncells", ctypes.c_size_t), ("cells", ctypes.POINTER(ctypes.c_int64)),
+                ("nsamples", ctypes.c_int), ("steps", ctypes.POINTER(ctypes.c_long)),
+                ("S_w", ctypes.POINTER(ctypes.c_double)), ("S_n", ctypes.POINTER(ctypes.c_double))]
+
+
+def make_observations(cells, steps, S_w, S_n):
+    """An Observations struct and the arrays it points into (keep both alive)."""
+    keep = [np.ascontiguousarray(cells, dtype=np.int64), np.ascontiguousarray(steps, dtype=np.int64),
+            np.ascontiguousarray(S_w, dtype=np.float64), np.ascontiguousarray(S_n, dtype=np.float64)]
+    o = Observations()
+    o.ncells = keep[0].size
+    o.cells = keep[0].ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    o.nsamples = keep[1].size
+    o.steps = keep[1].ctypes.data_as(ctypes.POINTER(ctypes.c_long))
+    o.S_w = keep[2].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    o.S_n = keep[3].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    return o, keep
 
 
 class Batch:

@@ -1,0 +1,407 @@
+// Discrete adjoint of the fused three-phase step (Corey closures), per cell.
+//
+// The forward step maps the state x_t (P_w, S_w, S_n in every cell; P_n, P_g
+// in the boundary cells, where they persist) and the parameters (K, m) to
+// x_{t+1}:
+//   interior l:  A_a(l) = roS_a(x_t(l)) - dtm(l) * div_a(l),
+//                x_{t+1}(l) = Newton(x_t(l), A(l))     (4 fixed iterations)
+//   boundary b:  x_{t+1}(b) = Border(x_{t+1}(inward(b)))
+// with div_a(l) the upwind 7-point flux divergence of phase a over the
+// coefficients (P_a, Xi_a, ro_a) of l and its six neighbours
+// (cpu.cpp:619-731) and dtm = dt / m.  Given the adjoint lambda_{t+1} of
+// x_{t+1}, the adjoint step returns lambda_t and adds the step's share of
+// dJ/d lnK and dJ/dm.  It runs in two passes over the cells:
+//
+//  1. adj_newton_cell (interior cells): folds the boundary cells' adjoints
+//     into their inward cell (the transpose of Border), recomputes A(l) and
+//     the Newton iterates, and runs the iteration backwards.  For
+//     x' = x - u, u = M(x)^-1 F(x, A) with M the reference's Newton matrix
+//     (three-phase.cpp:307-322, including its dF[4] = ro0_n*S_n entry) and
+//     J = dF/dx the true Jacobian:
+//         w = M^-T lambda',   lambda_A += w,
+//         lambda_j = lambda'_j - w . (J[:, j] - (dM/dx_j) u).
+//     It writes mu_a(l) = -dtm(l) * lambda_A_a(l), the adjoint reaching x_t(l)
+//     through the initial guess and roS, and the porosity gradient.
+//  2. adj_coef_cell (every cell): gathers mu over the transposed stencil --
+//     the cell's own divergence and the face terms of its interior
+//     neighbours that read its coefficients -- and chains the coefficients'
+//     adjoint to the state (P_a, S_w, S_n) and to ln K.
+//
+// The upwind switches are differentiated where they are smooth (the kink
+// at a zero driving force has measure zero).  Nothing here needs to match
+// the forward step bit for bit; the adjoint is checked against central
+// finite differences of the forward step.
+#pragma once
+
+#include "capaz_bodies.h"
+
+namespace capaz {
+
+// Adjoint pass arrays (device or host).
+struct AdjIO {
+    const double *Pw, *Sw, *Sn, *Pn, *Pg, *m, *K, *dtm;  // x_t and the parameters
+    const double *lPw1, *lSw1, *lSn1, *lPn1, *lPg1;       // lambda_{t+1}
+    double *mu_w, *mu_n, *mu_g;                           // pass 1 -> pass 2 (interior cells)
+    double *dPw, *dSw, *dSn;                              // pass 1 -> pass 2 (interior cells)
+    double *lPw, *lSw, *lSn, *lPn, *lPg;                  // lambda_t
+    double *g_lnK, *g_m;                                  // accumulated gradients
+};
+
+// Corey k_r and dk_r/ds at an effective saturation.
+CAPAZ_HD void corey_d(const DevParams& p, double s, double& kr, double& dkr) {
+    if (s <= 0.) {
+        kr = 0.;
+        dkr = 0.;
+    } else if (s >= 1.) {
+        kr = 1.;
+        dkr = 0.;
+    } else if (p.corey_n == 2.) {
+        kr = s * s;
+        dkr = 2. * s;
+    } else {
+        kr = pow(s, p.corey_n);
+        dkr = p.corey_n * pow(s, p.corey_n - 1.);
+    }
+}
+
+// A cell's coefficients and their derivatives with respect to its own state.
+struct CoefD {
+    double P[3], Xi[3], ro[3];
+    double dXi_dSw[3], dXi_dSn[3];
+    double dro_dP[3];
+    bool inner;  // P_n = P_g = P_w (three-phase.cpp:225-235)
+};
+
+CAPAZ_HD CoefD coef_d(const DevParams& p, const AdjIO& a, int64_t l, bool inner) {
+    CoefD c;
+    const double Pw = a.Pw[l], Sw = a.Sw[l], Sn = a.Sn[l], K = a.K[l];
+    c.inner = inner;
+    c.P[0] = Pw;
+    c.P[1] = inner ? Pw : a.Pn[l];
+    c.P[2] = inner ? Pw : a.Pg[l];
+    const double Swe = (Sw - p.S_wr) * p.r_den3, Sne = (Sn - p.S_nr) * p.r_den3;
+    const double Sge = 1. - Swe - Sne;
+    double kw, dkw, kn, dkn, kg, dkg;
+    corey_d(p, Swe, kw, dkw);
+    corey_d(p, Sne, kn, dkn);
+    corey_d(p, Sge, kg, dkg);
+    c.Xi[0] = -K * kw * p.r_mu_w;
+    c.Xi[1] = -K * kn * p.r_mu_n;
+    c.Xi[2] = -K * kg * p.r_mu_g;
+    c.dXi_dSw[0] = -K * dkw * p.r_mu_w * p.r_den3;
+    c.dXi_dSn[0] = 0.;
+    c.dXi_dSw[1] = 0.;
+    c.dXi_dSn[1] = -K * dkn * p.r_mu_n * p.r_den3;
+    c.dXi_dSw[2] = K * dkg * p.r_mu_g * p.r_den3;
+    c.dXi_dSn[2] = c.dXi_dSw[2];
+    c.ro[0] = p.ro0_w * (1. + p.beta_w * (c.P[0] - p.P_atm));
+    c.ro[1] = p.ro0_n * (1. + p.beta_n * (c.P[1] - p.P_atm));
+    c.ro[2] = p.ro0_g * c.P[2] * p.r_P_atm;
+    c.dro_dP[0] = p.c_wb;
+    c.dro_dP[1] = p.c_nb;
+    c.dro_dP[2] = p.ro0_g * p.r_P_atm;
+    return c;
+}
+
+// Values only (neighbours in the gathers).
+CAPAZ_HD Coef coef_v(const DevParams& p, const AdjIO& a, int64_t l, bool inner) {
+    Coef c;
+    const double Pw = a.Pw[l], Sw = a.Sw[l], Sn = a.Sn[l], K = a.K[l];
+    c.P[0] = Pw;
+    c.P[1] = inner ? Pw : a.Pn[l];
+    c.P[2] = inner ? Pw : a.Pg[l];
+    const double Swe = (Sw - p.S_wr) * p.r_den3, Sne = (Sn - p.S_nr) * p.r_den3;
+    double kw, kn, kg, d;
+    corey_d(p, Swe, kw, d);
+    corey_d(p, Sne, kn, d);
+    corey_d(p, 1. - Swe - Sne, kg, d);
+    c.Xi[0] = -K * kw * p.r_mu_w;
+    c.Xi[1] = -K * kn * p.r_mu_n;
+    c.Xi[2] = -K * kg * p.r_mu_g;
+    c.ro[0] = p.ro0_w * (1. + p.beta_w * (c.P[0] - p.P_atm));
+    c.ro[1] = p.ro0_n * (1. + p.beta_n * (c.P[1] - p.P_atm));
+    c.ro[2] = p.ro0_g * c.P[2] * p.r_P_atm;
+    return c;
+}
+
+// 3x3 solve M^T w = r by cofactors (M row-major).
+CAPAZ_HD void solve3_transposed(const double* M, const double* r, double* w) {
+    // M^T = [[M0, M3, M6], [M1, M4, M7], [M2, M5, M8]]
+    const double a = M[0], b = M[3], c = M[6];
+    const double d = M[1], e = M[4], f = M[7];
+    const double g = M[2], h = M[5], i = M[8];
+    const double A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
+    const double det = a * A + b * B + c * C;
+    const double rd = 1. / det;
+    const double D = -(b * i - c * h), E = a * i - c * g, F = -(a * h - b * g);
+    const double G = b * f - c * e, H = -(a * f - c * d), I = a * e - b * d;
+    // inverse of T = [[a,b,c],[d,e,f],[g,h,i]] is adj / det with adj = [[A, D, G], [B, E, H], [C, F, I]]
+    w[0] = (A * r[0] + D * r[1] + G * r[2]) * rd;
+    w[1] = (B * r[0] + E * r[1] + H * r[2]) * rd;
+    w[2] = (C * r[0] + F * r[1] + I * r[2]) * rd;
+}
+
+// The reference Newton matrix M (rows F1..F3, columns P, S_w, S_n) at x.
+CAPAZ_HD void newton3_matrix(const DevParams& p, double P, double Sw, double Sn, double* M) {
+    const double Sg = 1. - Sw - Sn;
+    const double gP = p.ro0_g * P * p.r_P_atm;
+    M[0] = p.c_wb * Sw;
+    M[1] = p.ro0_w * (1. + p.beta_w * (P - p.P_atm));
+    M[2] = 0.;
+    M[3] = p.c_nb * Sn;
+    M[4] = p.ro0_n * Sn;
+    M[5] = p.ro0_n * (1. + p.beta_n * (P - p.P_atm));
+    M[6] = p.ro0_g * Sg * p.r_P_atm;
+    M[7] = -gP;
+    M[8] = -gP;
+}
+
+// Adjoint of the fixed-iteration Newton (newton3) from (P0, Sw0, Sn0) with
+// right-hand sides A: given lam (adjoint of the result, overwritten with the
+// adjoint of the initial guess), adds the adjoint of A to lA.  The forward
+// iterates are recomputed first (the same iteration as newton3, solved by
+// cofactors).  IT > 0: that many iterations, unrolled so the iterates stay in
+// registers; IT == 0: p.iterations (at most kMaxAdjIter) in a loop.
+constexpr int kMaxAdjIter = 8;
+
+template <int IT>
+CAPAZ_HD void newton3_adjoint(const DevParams& p, double P0, double Sw0, double Sn0, const double* A, double* lam,
+                              double* lA) {
+    constexpr int NX = IT > 0 ? IT : kMaxAdjIter;
+    double xs[NX + 1][3];
+    const int it = IT > 0 ? IT : (p.iterations < kMaxAdjIter ? p.iterations : kMaxAdjIter);
+    xs[0][0] = P0;
+    xs[0][1] = Sw0;
+    xs[0][2] = Sn0;
+#pragma unroll
+    for (int k = 0; k < NX; ++k) {
+        if (k >= it) break;
+        const double P = xs[k][0], Sw = xs[k][1], Sn = xs[k][2];
+        double M[9];
+        newton3_matrix(p, P, Sw, Sn, M);
+        const double F[3] = {M[1] * Sw - A[0], M[5] * Sn - A[1], -M[7] * (1. - Sw - Sn) - A[2]};
+        // u = M^-1 F, as the transposed solve of M^T
+        double MT[9] = {M[0], M[3], M[6], M[1], M[4], M[7], M[2], M[5], M[8]};
+        double u[3];
+        solve3_transposed(MT, F, u);
+        xs[k + 1][0] = P - u[0];
+        xs[k + 1][1] = Sw - u[1];
+        xs[k + 1][2] = Sn - u[2];
+    }
+    const double rg = p.ro0_g * p.r_P_atm;
+#pragma unroll
+    for (int kk = NX - 1; kk >= 0; --kk) {
+        if (kk >= it) continue;
+        const int k = kk;
+        const double P = xs[k][0], Sw = xs[k][1], Sn = xs[k][2];
+        const double u0 = P - xs[k + 1][0], u1 = Sw - xs[k + 1][1], u2 = Sn - xs[k + 1][2];
+        double M[9];
+        newton3_matrix(p, P, Sw, Sn, M);
+        double w[3];
+        solve3_transposed(M, lam, w);
+        lA[0] += w[0];
+        lA[1] += w[1];
+        lA[2] += w[2];
+        // columns of J - (dM/dx_j) u, rows F1..F3
+        // j = P:   J = (c_wb Sw, c_nb Sn, rg Sg);  (dM/dP) u = (c_wb u1, c_nb u2, -rg (u1 + u2))
+        // j = S_w: J = (rw, 0, -gP);              (dM/dSw) u = (c_wb u0, 0, -rg u0)
+        // j = S_n: J = (0, rn, -gP);              (dM/dSn) u = (0, c_nb u0 + ro0_n u1, -rg u0)
+        const double gP = rg * P;
+        const double cP0 = M[0] - p.c_wb * u1, cP1 = M[3] - p.c_nb * u2, cP2 = M[6] + rg * (u1 + u2);
+        const double cW0 = M[1] - p.c_wb * u0, cW2 = -gP + rg * u0;
+        const double cN1 = M[5] - (p.c_nb * u0 + p.ro0_n * u1), cN2 = -gP + rg * u0;
+        lam[0] -= w[0] * cP0 + w[1] * cP1 + w[2] * cP2;
+        lam[1] -= w[0] * cW0 + w[2] * cW2;
+        lam[2] -= w[1] * cN1 + w[2] * cN2;
+    }
+}
+
+// Boundary cells whose inward neighbour is the interior cell (i, j, k), as
+// boundary_epilogue enumerates them; f(bi, bj, bk) for each.
+template <class Fn>
+CAPAZ_HD void for_boundary_of(const DevParams& p, int i, int j, int k, Fn&& f) {
+    int ox[3] = {0, 0, 0}, oy[3] = {0, 0, 0}, oz[3] = {0, 0, 0};
+    int nx = 1, ny = 1, nz = 1;
+    if (p.Nx > 2 && i == 1) ox[nx++] = -1;
+    if (p.Nx > 2 && i == p.Nx - 2) ox[nx++] = +1;
+    if (j == 1) oy[ny++] = -1;
+    if (j == p.Ny - 2) oy[ny++] = +1;
+    if (p.Nz > 2 && k == 1) oz[nz++] = -1;
+    if (p.Nz > 2 && k == p.Nz - 2) oz[nz++] = +1;
+    for (int a = 0; a < nz; ++a)
+        for (int b = 0; b < ny; ++b)
+            for (int c = 0; c < nx; ++c) {
+                if (a == 0 && b == 0 && c == 0) continue;
+                f(i + ox[c], j + oy[b], k + oz[a]);
+            }
+}
+
+// Pass 1 at interior cell (i, j, k), l its index (IT: newton3_adjoint's).
+template <int IT>
+CAPAZ_HD void adj_newton_cell(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l) {
+    // lambda of x_{t+1}(l), plus the boundary cells copied from it
+    double lam[3] = {a.lPw1[l], a.lSw1[l], a.lSn1[l]};
+    for_boundary_of(p, i, j, k, [&](int bi, int bj, int bk) {
+        const int64_t lb = (int64_t)bi + (int64_t)bj * p.sy + (int64_t)bk * p.sz;
+        double dw = 1., dn = 1., dg = 1.;  // d(P_w, P_n, P_g)_b / dP_w(l), border_p3
+        if (bj == 0) {
+            dw = p.r_bp_w_top_den;
+            dn = p.r_bp_n_top_den;
+            dg = p.r_bp_g_top_den;
+        } else if (bj == p.Ny - 1) {
+            dw = p.r_bp_w_bot_den;
+            dn = p.r_bp_n_bot_den;
+            dg = p.r_bp_g_bot_den;
+        }
+        lam[0] += dw * a.lPw1[lb] + dn * a.lPn1[lb] + dg * a.lPg1[lb];
+        if (!(bj == 0 && p.source > 0)) {
+            lam[1] += a.lSw1[lb];
+            lam[2] += a.lSn1[lb];
+        }
+    });
+    // forward quantities of the step at l
+    const Coef c = coef_v(p, a, l, true);
+    auto nb = [&](int ii, int jj, int kk) {
+        const int64_t q = (int64_t)ii + (int64_t)jj * p.sy + (int64_t)kk * p.sz;
+        return coef_v(p, a, q, interior3(p, ii, jj, kk));
+    };
+    Coef xm = c, xp = c, zm = c, zp = c;
+    if (!(p.Nx < 2)) {
+        xm = nb(i - 1, j, k);
+        xp = nb(i + 1, j, k);
+    }
+    if (!(p.Nz < 2)) {
+        zm = nb(i, j, k - 1);
+        zp = nb(i, j, k + 1);
+    }
+    const Coef ym = nb(i, j - 1, k), yp = nb(i, j + 1, k);
+    const double Sw = a.Sw[l], Sn = a.Sn[l], Sg = 1. - Sw - Sn, dtm = a.dtm[l];
+    double div[3], A[3];
+    for (int ph = 0; ph < 3; ++ph) div[ph] = coef_divergence(p, ph, c, xm, xp, ym, yp, zm, zp);
+    A[0] = c.ro[0] * Sw - dtm * div[0];
+    A[1] = c.ro[1] * Sn - dtm * div[1];
+    A[2] = c.ro[2] * Sg - dtm * div[2];
+    double lA[3] = {0., 0., 0.};
+    newton3_adjoint<IT>(p, a.Pw[l], Sw, Sn, A, lam, lA);
+    a.mu_w[l] = -dtm * lA[0];
+    a.mu_n[l] = -dtm * lA[1];
+    a.mu_g[l] = -dtm * lA[2];
+    // dA/dm = div * dtm / m
+    a.g_m[l] += (lA[0] * div[0] + lA[1] * div[1] + lA[2] * div[2]) * dtm / a.m[l];
+    // roS_a(x_t(l)) and the Newton initial guess
+    const double rg = p.ro0_g * p.r_P_atm;
+    a.dPw[l] = lam[0] + lA[0] * p.c_wb * Sw + lA[1] * p.c_nb * Sn + lA[2] * rg * Sg;
+    a.dSw[l] = lam[1] + lA[0] * c.ro[0] - lA[2] * c.ro[2];
+    a.dSn[l] = lam[2] + lA[1] * c.ro[1] - lA[2] * c.ro[2];
+}
+
+// Face-term derivatives of directed() for one cell's flux along one axis.
+// x1, x2: the cell's minus / plus driving forces; L = -Xi*ro of the cell (c)
+// and of its minus (m) / plus (p) neighbours; rh: 1 / the axis spacing.
+struct DirD {
+    double dx1, dx2, dLc, dLm, dLp;
+};
+CAPAZ_HD DirD directed_d(double x1, double x2, double Lc, double Lm, double Lp, double rh) {
+    DirD d;
+    d.dx1 = (-(x1 < 0. ? Lc : 0.) - (x1 > 0. ? Lm : 0.)) * rh;
+    d.dx2 = ((x2 > 0. ? Lc : 0.) + (x2 < 0. ? Lp : 0.)) * rh;
+    d.dLc = ((x2 > 0. ? x2 : 0.) - (x1 < 0. ? x1 : 0.)) * rh;
+    d.dLm = -(x1 > 0. ? x1 : 0.) * rh;
+    d.dLp = (x2 < 0. ? x2 : 0.) * rh;
+    return d;
+}
+
+// Pass 2 at cell (i, j, k), l its index: lambda_t(l) and the ln K gradient.
+CAPAZ_HD void adj_coef_cell(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l) {
+    const bool inner = interior3(p, i, j, k);
+    const CoefD c = coef_d(p, a, l, inner);
+    double lP[3] = {0., 0., 0.}, lL[3] = {0., 0., 0.}, lro[3] = {0., 0., 0.};
+    const double g = p.g;
+    // axis: stride, spacing, active, gravity
+    for (int ax = 0; ax < 3; ++ax) {
+        int di = 0, dj = 0, dk = 0, n = 0, pos = 0;
+        double rh;
+        bool grav = false;
+        if (ax == 0) {
+            if (p.Nx < 2) continue;
+            di = 1; n = p.Nx; pos = i; rh = p.r_hx;
+        } else if (ax == 1) {
+            dj = 1; n = p.Ny; pos = j; rh = p.r_hy; grav = true;
+        } else {
+            if (p.Nz < 2) continue;
+            dk = 1; n = p.Nz; pos = k; rh = p.r_hz;
+        }
+        const bool has_m = pos - 1 >= 0, has_p = pos + 1 < n;
+        Coef cm{}, cp{};
+        bool m_in = false, p_in = false;
+        if (has_m) {
+            m_in = interior3(p, i - di, j - dj, k - dk);
+            cm = coef_v(p, a, l - (int64_t)di - (int64_t)dj * p.sy - (int64_t)dk * p.sz, m_in);
+        }
+        if (has_p) {
+            p_in = interior3(p, i + di, j + dj, k + dk);
+            cp = coef_v(p, a, l + (int64_t)di + (int64_t)dj * p.sy + (int64_t)dk * p.sz, p_in);
+        }
+        const int64_t qm = l - (int64_t)di - (int64_t)dj * p.sy - (int64_t)dk * p.sz;
+        const int64_t qp = l + (int64_t)di + (int64_t)dj * p.sy + (int64_t)dk * p.sz;
+        for (int ph = 0; ph < 3; ++ph) {
+            const double mu_self = inner ? (ph == 0 ? a.mu_w[l] : ph == 1 ? a.mu_n[l] : a.mu_g[l]) : 0.;
+            const double Lc = -c.Xi[ph] * c.ro[ph];
+            if (inner) {  // the cell's own flux along this axis (both neighbours exist)
+                const double gr = grav ? g * c.ro[ph] : 0.;
+                const double x1 = -((c.P[ph] - cm.P[ph]) * rh) + gr, x2 = -((cp.P[ph] - c.P[ph]) * rh) + gr;
+                const DirD d = directed_d(x1, x2, Lc, -cm.Xi[ph] * cm.ro[ph], -cp.Xi[ph] * cp.ro[ph], rh);
+                lP[ph] += mu_self * (d.dx2 - d.dx1) * rh;  // dx1/dP_c = -1/h, dx2/dP_c = +1/h
+                lL[ph] += mu_self * d.dLc;
+                if (grav) lro[ph] += mu_self * (d.dx1 + d.dx2) * g;
+            }
+            if (has_p && p_in) {  // plus neighbour's flux: this cell is its minus neighbour
+                const double mu = ph == 0 ? a.mu_w[qp] : ph == 1 ? a.mu_n[qp] : a.mu_g[qp];
+                const double gr = grav ? g * cp.ro[ph] : 0.;
+                const double x1 = -((cp.P[ph] - c.P[ph]) * rh) + gr;
+                const double Lq = -cp.Xi[ph] * cp.ro[ph];
+                const double dx1 = (-(x1 < 0. ? Lq : 0.) - (x1 > 0. ? Lc : 0.)) * rh;
+                lP[ph] += mu * dx1 * rh;  // dx1/dP_m = +1/h
+                lL[ph] += mu * (-(x1 > 0. ? x1 : 0.) * rh);
+            }
+            if (has_m && m_in) {  // minus neighbour's flux: this cell is its plus neighbour
+                const double mu = ph == 0 ? a.mu_w[qm] : ph == 1 ? a.mu_n[qm] : a.mu_g[qm];
+                const double gr = grav ? g * cm.ro[ph] : 0.;
+                const double x2 = -((c.P[ph] - cm.P[ph]) * rh) + gr;
+                const double Lq = -cm.Xi[ph] * cm.ro[ph];
+                const double dx2 = ((x2 > 0. ? Lq : 0.) + (x2 < 0. ? Lc : 0.)) * rh;
+                lP[ph] += mu * (-dx2 * rh);  // dx2/dP_p = -1/h
+                lL[ph] += mu * ((x2 < 0. ? x2 : 0.) * rh);
+            }
+        }
+    }
+    // coefficients -> state and ln K
+    double lPw = 0., lSw = 0., lSn = 0., lPn = 0., lPg = 0., glnK = 0.;
+    for (int ph = 0; ph < 3; ++ph) {
+        const double lXi = -lL[ph] * c.ro[ph];
+        const double lr = lro[ph] - lL[ph] * c.Xi[ph];
+        const double lp = lP[ph] + lr * c.dro_dP[ph];
+        if (ph == 0 || c.inner)
+            lPw += lp;
+        else if (ph == 1)
+            lPn += lp;
+        else
+            lPg += lp;
+        lSw += lXi * c.dXi_dSw[ph];
+        lSn += lXi * c.dXi_dSn[ph];
+        glnK += lXi * c.Xi[ph];  // dXi/dlnK = Xi
+    }
+    if (inner) {
+        lPw += a.dPw[l];
+        lSw += a.dSw[l];
+        lSn += a.dSn[l];
+    }
+    a.lPw[l] = lPw;
+    a.lSw[l] = lSw;
+    a.lSn[l] = lSn;
+    a.lPn[l] = lPn;
+    a.lPg[l] = lPg;
+    a.g_lnK[l] += glnK;
+}
+
+}  // namespace capaz

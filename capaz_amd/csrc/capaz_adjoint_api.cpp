@@ -1,0 +1,290 @@
+// C ABI of the history-matching objective and its discrete adjoint
+// (include/capaz.h: capaz_objective, capaz_adjoint).
+//
+// capaz_adjoint runs the forward steps from the context's state, keeping a
+// checkpoint of the primary state every C steps (C = round(sqrt(nsteps)) by
+// default) and the observed saturations at the sampled steps, then sweeps the
+// segments backwards: each segment is recomputed from its checkpoint with
+// the fused kernel (the same bits as the first pass), and its steps are
+// reversed with the two adjoint passes.  Device memory: (nsteps/C + C) states
+// of five fields plus 18 fields of adjoint work, held by the context between
+// calls.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "capaz_adjoint_launch.h"
+#include "capaz_ctx.h"
+
+namespace {
+
+using capaz::detail::fail;
+using capaz::detail::hip_fail;
+
+#define ADJ_HIP(c, call)                                      \
+    do {                                                      \
+        hipError_t e_ = (call);                               \
+        if (e_ != hipSuccess) return hip_fail((c), e_, #call); \
+    } while (0)
+
+constexpr int kState = 5;  // P_w, S_w, S_n, P_n, P_g
+const int kStateField[kState] = {capaz::F_P_w, capaz::F_S_w, capaz::F_S_n, capaz::F_P_n, capaz::F_P_g};
+
+struct Obs {
+    std::vector<int> sample_of;  // step -> sample index or -1
+    int64_t ncells = 0;
+    int nsamples = 0;
+};
+
+int check_model(capaz_ctx* c) {
+    if (c->user.model != 3 || c->user.energy || c->user.closure != CAPAZ_CLOSURE_COREY)
+        return fail(c, CAPAZ_ERR_INVALID, "adjoint: three-phase model with the Corey closure only");
+    if (c->p.iterations > capaz::kMaxAdjIter)
+        return fail(c, CAPAZ_ERR_INVALID, "adjoint: at most 8 Newton iterations");
+    if (c->p.Ny > 65535 || c->p.Nz > 65535) return fail(c, CAPAZ_ERR_INVALID, "adjoint: Ny, Nz <= 65535");
+    return CAPAZ_OK;
+}
+
+int check_obs(capaz_ctx* c, long nsteps, const capaz_observations* o, Obs* out) {
+    if (nsteps < 1) return fail(c, CAPAZ_ERR_INVALID, "nsteps must be >= 1");
+    if (!o) return fail(c, CAPAZ_ERR_INVALID, "NULL observations");
+    if (o->nsamples < 0) return fail(c, CAPAZ_ERR_INVALID, "observations: nsamples < 0");
+    if ((o->ncells && !o->cells) || (o->nsamples && (!o->steps || (o->ncells && (!o->S_w || !o->S_n)))))
+        return fail(c, CAPAZ_ERR_INVALID, "observations: NULL array");
+    out->sample_of.assign((size_t)nsteps + 1, -1);
+    long prev = 0;
+    for (int s = 0; s < o->nsamples; ++s) {
+        if (o->steps[s] <= prev || o->steps[s] > nsteps)
+            return fail(c, CAPAZ_ERR_INVALID, "observations: steps must increase within 1..nsteps");
+        prev = o->steps[s];
+        out->sample_of[(size_t)o->steps[s]] = s;
+    }
+    std::vector<int64_t> sorted(o->cells, o->cells + o->ncells);
+    std::sort(sorted.begin(), sorted.end());
+    for (size_t q = 0; q < sorted.size(); ++q) {
+        if (sorted[q] < 0 || sorted[q] >= c->p.n) return fail(c, CAPAZ_ERR_INVALID, "observations: cell out of range");
+        if (q && sorted[q] == sorted[q - 1]) return fail(c, CAPAZ_ERR_INVALID, "observations: repeated cell");
+    }
+    out->ncells = (int64_t)o->ncells;
+    out->nsamples = o->nsamples;
+    return CAPAZ_OK;
+}
+
+// Device workspace of the context, grown on demand (fields of `stride` doubles).
+int workspace(capaz_ctx* c, size_t doubles, double** out) {
+    const size_t bytes = doubles * sizeof(double);
+    if (c->adj_bytes < bytes) {
+        ADJ_HIP(c, hipStreamSynchronize(c->stream));
+        if (c->adj_mem) (void)hipFree(c->adj_mem);
+        c->adj_mem = nullptr;
+        c->adj_bytes = 0;
+        ADJ_HIP(c, hipMalloc(&c->adj_mem, bytes));
+        c->adj_bytes = bytes;
+    }
+    *out = c->adj_mem;
+    return CAPAZ_OK;
+}
+
+// One fused step from the five arrays at `from` to those at `to` (stride apart).
+int fused_between(capaz_ctx* c, const double* from, double* to, size_t stride) {
+    capaz::StepIO io;
+    io.Pw = from;
+    io.Sw = from + stride;
+    io.Sn = from + 2 * stride;
+    io.Pn = from + 3 * stride;
+    io.Pg = from + 4 * stride;
+    io.m = c->cur[capaz::F_m];
+    io.K = c->cur[capaz::F_K];
+    io.dtm = c->d_dtm;
+    io.Pw1 = to;
+    io.Sw1 = to + stride;
+    io.Sn1 = to + 2 * stride;
+    io.Pn1 = to + 3 * stride;
+    io.Pg1 = to + 4 * stride;
+    ADJ_HIP(c, capaz::launch_fused(c->p, io, c->tuning, c->stream));
+    return CAPAZ_OK;
+}
+
+int copy_state(capaz_ctx* c, double* dst, size_t stride) {
+    const size_t bytes = (size_t)c->p.n * sizeof(double);
+    for (int f = 0; f < kState; ++f)
+        ADJ_HIP(c, hipMemcpyAsync(dst + f * stride, c->cur[kStateField[f]], bytes, hipMemcpyDeviceToDevice, c->stream));
+    return CAPAZ_OK;
+}
+
+// Forward steps from the context's state with the observation gathers;
+// every `every` steps (if > 0) the state is copied to ckpt + q*5*stride.
+// seen: [nsamples][2][ncells] on the device.
+int forward(capaz_ctx* c, long nsteps, const Obs& ob, const int64_t* d_cells, double* seen, long every, double* ckpt,
+            size_t stride) {
+    int rc;
+    if (every > 0 && (rc = copy_state(c, ckpt, stride))) return rc;
+    for (long t = 0; t < nsteps; ++t) {
+        if ((rc = capaz_step(c, t + 1, 1))) return rc;
+        const long t1 = t + 1;
+        if (every > 0 && t1 % every == 0 && t1 < nsteps &&
+            (rc = copy_state(c, ckpt + (size_t)(t1 / every) * kState * stride, stride)))
+            return rc;
+        const int s = ob.sample_of[(size_t)t1];
+        if (s >= 0)
+            ADJ_HIP(c, capaz::launch_obs_gather(c->cur[capaz::F_S_w], c->cur[capaz::F_S_n], d_cells, ob.ncells,
+                                                seen + (size_t)s * 2 * ob.ncells,
+                                                seen + (size_t)s * 2 * ob.ncells + ob.ncells, c->stream));
+    }
+    return CAPAZ_OK;
+}
+
+// 0.5 * sum of squared mismatches, in sample-then-cell order.
+int objective_value(capaz_ctx* c, const capaz_observations* o, const Obs& ob, const double* seen, double* J) {
+    std::vector<double> h((size_t)ob.nsamples * 2 * ob.ncells);
+    if (!h.empty())
+        ADJ_HIP(c, hipMemcpyAsync(h.data(), seen, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    ADJ_HIP(c, hipStreamSynchronize(c->stream));
+    double sum = 0.;
+    for (int s = 0; s < ob.nsamples; ++s)
+        for (int64_t q = 0; q < ob.ncells; ++q) {
+            const double dw = h[(size_t)s * 2 * ob.ncells + q] - o->S_w[(size_t)s * ob.ncells + q];
+            const double dn = h[(size_t)s * 2 * ob.ncells + ob.ncells + q] - o->S_n[(size_t)s * ob.ncells + q];
+            sum += dw * dw + dn * dn;
+        }
+    *J = 0.5 * sum;
+    return CAPAZ_OK;
+}
+
+size_t field_stride(int64_t n) { return ((size_t)n + 31) & ~(size_t)31; }  // 256-byte aligned fields
+
+}  // namespace
+
+extern "C" {
+
+int capaz_objective(capaz_ctx* c, long nsteps, const capaz_observations* o, double* J) {
+    int rc;
+    if (!c || !J) return CAPAZ_ERR_INVALID;
+    if ((rc = capaz::detail::check_ctx(c))) return rc;
+    if ((rc = check_model(c))) return rc;
+    Obs ob;
+    if ((rc = check_obs(c, nsteps, o, &ob))) return rc;
+    if ((rc = capaz::detail::scan_m(c))) return rc;
+    if (c->m_has_zero) return fail(c, CAPAZ_ERR_STATE, "adjoint: cells with m == 0");
+    const size_t ns = (size_t)ob.nsamples * 2 * ob.ncells;
+    double* w;
+    if ((rc = workspace(c, ns + (size_t)ob.ncells, &w))) return rc;
+    int64_t* d_cells = reinterpret_cast<int64_t*>(w + ns);
+    if (ob.ncells)
+        ADJ_HIP(c, hipMemcpyAsync(d_cells, o->cells, ob.ncells * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    if ((rc = forward(c, nsteps, ob, d_cells, w, 0, nullptr, 0))) return rc;
+    return objective_value(c, o, ob, w, J);
+}
+
+int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int checkpoint_every, double* J,
+                  double* grad_lnK, double* grad_m) {
+    using namespace capaz;
+    int rc;
+    if (!c || !J) return CAPAZ_ERR_INVALID;
+    if ((rc = detail::check_ctx(c))) return rc;
+    if ((rc = check_model(c))) return rc;
+    Obs ob;
+    if ((rc = check_obs(c, nsteps, o, &ob))) return rc;
+    if ((rc = detail::scan_m(c))) return rc;
+    if (c->m_has_zero) return fail(c, CAPAZ_ERR_STATE, "adjoint: cells with m == 0");
+    long C = checkpoint_every > 0 ? checkpoint_every : std::lround(std::sqrt((double)nsteps));
+    C = std::max(1L, std::min(C, nsteps));
+    const long nseg = (nsteps + C - 1) / C;
+    const size_t S = field_stride(c->p.n);
+    const size_t n_state = (size_t)kState * S;
+    // layout: checkpoints | segment states | lambda x2 | work (6) | grads (2) | seen | observed | cells
+    const size_t ns = (size_t)ob.nsamples * 2 * ob.ncells;
+    const size_t total =
+        (size_t)nseg * n_state + (size_t)C * n_state + 2 * n_state + 8 * S + 2 * ns + (size_t)ob.ncells;
+    double* w;
+    if ((rc = workspace(c, total, &w))) return rc;
+    double* ckpt = w;
+    double* seg = ckpt + (size_t)nseg * n_state;
+    double* lam[2] = {seg + (size_t)C * n_state, seg + (size_t)C * n_state + n_state};
+    double* work = lam[1] + n_state;
+    double* grad = work + 6 * S;
+    double* seen = grad + 2 * S;
+    double* d_want = seen + ns;
+    int64_t* d_cells = reinterpret_cast<int64_t*>(d_want + ns);
+    if (ob.ncells)
+        ADJ_HIP(c, hipMemcpyAsync(d_cells, o->cells, ob.ncells * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    std::vector<double> want(ns);  // the observed values, laid out as `seen`
+    for (int s = 0; s < ob.nsamples; ++s)
+        for (int64_t q = 0; q < ob.ncells; ++q) {
+            want[(size_t)s * 2 * ob.ncells + q] = o->S_w[(size_t)s * ob.ncells + q];
+            want[(size_t)s * 2 * ob.ncells + ob.ncells + q] = o->S_n[(size_t)s * ob.ncells + q];
+        }
+    if (ns) ADJ_HIP(c, hipMemcpyAsync(d_want, want.data(), ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    // forward sweep (objective_value waits for it, so `want` outlives its copy)
+    if ((rc = forward(c, nsteps, ob, d_cells, seen, C, ckpt, S))) return rc;
+    if ((rc = objective_value(c, o, ob, seen, J))) return rc;
+    auto obs_adjoint = [&](double* l, long step) -> int {
+        const int s = ob.sample_of[(size_t)step];
+        if (s < 0) return CAPAZ_OK;
+        const size_t off = (size_t)s * 2 * ob.ncells;
+        hipError_t e = launch_obs_adjoint(l + 1 * S, l + 2 * S, d_cells, ob.ncells, seen + off, seen + off + ob.ncells,
+                                          d_want + off, d_want + off + ob.ncells, c->stream);
+        return e == hipSuccess ? CAPAZ_OK : hip_fail(c, e, "launch_obs_adjoint");
+    };
+    // reverse sweep
+    hipError_t e;
+    if ((e = hipMemsetAsync(lam[0], 0, n_state * sizeof(double), c->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(grad, 0, 2 * S * sizeof(double), c->stream)) != hipSuccess)
+        return hip_fail(c, e, "hipMemsetAsync");
+    if ((rc = obs_adjoint(lam[0], nsteps))) return rc;
+    int cur = 0;
+    for (long q = nseg - 1; q >= 0; --q) {
+        const long t0 = q * C, t1 = std::min(nsteps, t0 + C);
+        auto slot = [&](long t) { return t == t0 ? ckpt + (size_t)q * n_state : seg + (size_t)(t - t0) * n_state; };
+        for (long t = t0; t + 1 < t1; ++t)
+            if ((rc = fused_between(c, slot(t), slot(t + 1), S))) return rc;
+        for (long t = t1 - 1; t >= t0; --t) {
+            const double* x = slot(t);
+            AdjIO a;
+            a.Pw = x;
+            a.Sw = x + S;
+            a.Sn = x + 2 * S;
+            a.Pn = x + 3 * S;
+            a.Pg = x + 4 * S;
+            a.m = c->cur[F_m];
+            a.K = c->cur[F_K];
+            a.dtm = c->d_dtm;
+            const double* l1 = lam[cur];
+            double* l0 = lam[cur ^ 1];
+            a.lPw1 = l1;
+            a.lSw1 = l1 + S;
+            a.lSn1 = l1 + 2 * S;
+            a.lPn1 = l1 + 3 * S;
+            a.lPg1 = l1 + 4 * S;
+            a.mu_w = work;
+            a.mu_n = work + S;
+            a.mu_g = work + 2 * S;
+            a.dPw = work + 3 * S;
+            a.dSw = work + 4 * S;
+            a.dSn = work + 5 * S;
+            a.lPw = l0;
+            a.lSw = l0 + S;
+            a.lSn = l0 + 2 * S;
+            a.lPn = l0 + 3 * S;
+            a.lPg = l0 + 4 * S;
+            a.g_lnK = grad;
+            a.g_m = grad + S;
+            if ((e = launch_adjoint_step(c->p, a, c->stream)) != hipSuccess)
+                return hip_fail(c, e, "launch_adjoint_step");
+            if (t >= 1 && (rc = obs_adjoint(l0, t))) return rc;
+            cur ^= 1;
+        }
+    }
+    const size_t bytes = (size_t)c->p.n * sizeof(double);
+    if (grad_lnK && (e = hipMemcpyAsync(grad_lnK, grad, bytes, hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
+        return hip_fail(c, e, "hipMemcpyAsync");
+    if (grad_m && (e = hipMemcpyAsync(grad_m, grad + S, bytes, hipMemcpyDeviceToHost, c->stream)) != hipSuccess)
+        return hip_fail(c, e, "hipMemcpyAsync");
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return hip_fail(c, e, "hipStreamSynchronize");
+    return CAPAZ_OK;
+}
+
+}  // extern "C"

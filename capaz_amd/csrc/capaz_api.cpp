@@ -232,6 +232,7 @@ int capaz_destroy(capaz_ctx* c) {
     }
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->d_dtm) (void)hipFree(c->d_dtm);
+    if (c->adj_mem) (void)hipFree(c->adj_mem);
     for (auto& e : c->ev) {
         if (e.begin) (void)hipEventDestroy(e.begin);
         if (e.end) (void)hipEventDestroy(e.end);

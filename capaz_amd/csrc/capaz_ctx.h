@@ -30,6 +30,9 @@ struct capaz_ctx {
     size_t ev_used = 0;
     double timed_ms = 0.;
     long timed_launches = 0;
+    // adjoint workspace (checkpoints, segment states, adjoint fields)
+    double* adj_mem = nullptr;
+    size_t adj_bytes = 0;
     std::string err;
 };
 

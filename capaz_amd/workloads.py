@@ -116,3 +116,31 @@ def ensemble(count: int = 64, seed: int = 1003, dt: float = 1e-6):
         shape = (n, n, n)
         out.append(dict(dims=(n, n, n), P_w=np.full(shape, keys["P_ATM"]), S_n=np.full(shape, 0.85), m=m, K=K))
     return keys, out
+
+
+def history_matching(n: int = 96, nsteps: int = 400, seed: int = 1004, columns: int = 16, every: int = 50):
+    """Permeability inversion on an n^3 three-phase grid (physics of
+    three_phase_injection): a seeded "true" ln K (ln-sigma 1.0 around
+    ln 6.64e-11, 6-cell Gaussian correlation), the trial ln K = its mean
+    everywhere, porosity 0.4(1 + 0.05 N(0,1)) clipped to [0.3, 0.5] for both,
+    and `columns` seeded observation columns (i, k) whose interior cells
+    (j = 1 .. n-2) are observed every `every` steps up to nsteps.
+
+    dt = 1e-6 s: the true field's K tails reach ~100 K_mean, and dt = 1e-5
+    diverges within 50 steps at 96^3 (dt = 1e-6 runs the 400 steps).
+
+    Returns (keys, state, lnK_true, lnK_trial, obs) with state the initial
+    primary fields and porosity, obs = {cells, steps} (the observed values
+    come from a forward run with lnK_true)."""
+    keys, st = three_phase_injection(n, seed=seed)
+    keys = dict(keys, DT=1e-6)
+    rng = np.random.default_rng(seed + 1)
+    lnK_true = _correlated_lnK(rng, n, 6.0, 1.0)
+    lnK_trial = np.full((n, n, n), np.log(K_MEAN))
+    st.pop("K")
+    ik = rng.choice((n - 2) * (n - 2), size=columns, replace=False)
+    i_c, k_c = 1 + ik % (n - 2), 1 + ik // (n - 2)
+    j = np.arange(1, n - 1)
+    cells = (i_c[:, None] + j[None, :] * n + k_c[:, None] * n * n).ravel().astype(np.int64)
+    steps = np.arange(every, nsteps + 1, every, dtype=np.int64)
+    return keys, st, lnK_true, lnK_trial, {"cells": cells, "steps": steps}

@@ -26,6 +26,7 @@
 #define CAPAZ_H
 
 #include <stddef.h>
+#include <stdint.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -188,6 +189,34 @@ int capaz_stream(capaz_ctx* ctx, void** stream);
  * plane's size; the plane is laid out as the reference's HostBuffer. */
 int capaz_load_face(capaz_ctx* ctx, int field, int axis, int side, double* dbuf);
 int capaz_save_face(capaz_ctx* ctx, int field, int axis, int side, const double* dbuf);
+
+/* ---- History matching: objective and discrete adjoint ----------------------
+ * No counterpart in the reference.  The objective of nsteps fused steps from
+ * the context's current state is
+ *     J = 0.5 * sum_s sum_c [(S_w(c, steps[s]) - S_w_obs[s][c])^2
+ *                            + (S_n(c, steps[s]) - S_n_obs[s][c])^2]
+ * over the observed cells c and steps.  Three-phase model, Corey closure, at
+ * most 8 Newton iterations, no m == 0 cells. */
+typedef struct capaz_observations {
+    size_t ncells;
+    const int64_t* cells;  /* distinct linear cell indices i + j*Nx + k*Nx*Ny */
+    int nsamples;
+    const long* steps;     /* increasing step counts in 1..nsteps */
+    const double* S_w;     /* nsamples x ncells, row-major */
+    const double* S_n;     /* nsamples x ncells, row-major */
+} capaz_observations;
+
+/* J after nsteps fused steps; the context then holds the state after them
+ * (as capaz_step would). */
+int capaz_objective(capaz_ctx* ctx, long nsteps, const capaz_observations* obs, double* J);
+/* J and its gradient with respect to ln K and m of every cell (host arrays
+ * of Nx*Ny*Nz doubles, either may be NULL), by the hand-written discrete
+ * adjoint of the fused step, with a checkpoint of the state every
+ * checkpoint_every steps (<= 0: round(sqrt(nsteps))) and each segment
+ * recomputed from its checkpoint on the way back.  The context then holds the
+ * state after nsteps, as after capaz_objective. */
+int capaz_adjoint(capaz_ctx* ctx, long nsteps, const capaz_observations* obs, int checkpoint_every, double* J,
+                  double* grad_lnK, double* grad_m);
 
 /* ---- Batched ensembles ------------------------------------------------------
  * No counterpart in the reference (one domain per process there): an

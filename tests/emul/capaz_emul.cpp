@@ -7,6 +7,7 @@
 // (glibc here, the ROCm device library on the GPU).
 #include <cstdint>
 
+#include "capaz_adjoint.h"
 #include "capaz_bodies.h"
 #include "capaz_derive.h"
 
@@ -48,6 +49,33 @@ int capaz_emul_fused_step(const capaz_params* u, double** cur, double** next) {
     io.Pw = cur[0]; io.Sw = cur[1]; io.Sn = cur[2]; io.Pn = cur[3]; io.Pg = cur[4]; io.m = cur[5]; io.K = cur[6];
     io.Pw1 = next[0]; io.Sw1 = next[1]; io.Sn1 = next[2]; io.Pn1 = next[3]; io.Pg1 = next[4];
     for (int64_t l = 0; l < p.n; ++l) capaz::body_fused_step(p, io, l);
+    return 0;
+}
+
+// One adjoint step (capaz_adjoint.h) on host arrays:
+// state = {Pw, Sw, Sn, Pn, Pg, m, K, dtm} at t, lam1 = lambda_{t+1}
+// {Pw, Sw, Sn, Pn, Pg}, work = {mu_w, mu_n, mu_g, dPw, dSw, dSn} (scratch),
+// lam = lambda_t (written), grad = {g_lnK, g_m} (accumulated).
+int capaz_emul_adjoint_step(const capaz_params* u, double** state, double** lam1, double** work, double** lam,
+                            double** grad) {
+    if (!capaz::validate(*u).empty()) return -1;
+    capaz::DevParams p = capaz::derive(*u);
+    capaz::AdjIO a;
+    a.Pw = state[0]; a.Sw = state[1]; a.Sn = state[2]; a.Pn = state[3]; a.Pg = state[4];
+    a.m = state[5]; a.K = state[6]; a.dtm = state[7];
+    a.lPw1 = lam1[0]; a.lSw1 = lam1[1]; a.lSn1 = lam1[2]; a.lPn1 = lam1[3]; a.lPg1 = lam1[4];
+    a.mu_w = work[0]; a.mu_n = work[1]; a.mu_g = work[2]; a.dPw = work[3]; a.dSw = work[4]; a.dSn = work[5];
+    a.lPw = lam[0]; a.lSw = lam[1]; a.lSn = lam[2]; a.lPn = lam[3]; a.lPg = lam[4];
+    a.g_lnK = grad[0]; a.g_m = grad[1];
+    for (int k = 0; k < p.Nz; ++k)
+        for (int j = 0; j < p.Ny; ++j)
+            for (int i = 0; i < p.Nx; ++i)
+                if (capaz::interior3(p, i, j, k))
+                    capaz::adj_newton_cell<0>(p, a, i, j, k, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz);
+    for (int k = 0; k < p.Nz; ++k)
+        for (int j = 0; j < p.Ny; ++j)
+            for (int i = 0; i < p.Nx; ++i)
+                capaz::adj_coef_cell(p, a, i, j, k, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz);
     return 0;
 }
 

@@ -34,6 +34,7 @@ def lib():
         L.capaz_emul_entry.argtypes = [ctypes.POINTER(Params), PP, ctypes.c_int]
         L.capaz_emul_fused_step.argtypes = [ctypes.POINTER(Params), PP, PP]
         L.capaz_emul_reverse_matrix.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+        L.capaz_emul_adjoint_step.argtypes = [ctypes.POINTER(Params), PP, PP, PP, PP, PP]
         L.capaz_emul_validate.argtypes = [ctypes.POINTER(Params)]
         L.capaz_emul_validate.restype = ctypes.c_char_p
         _lib = L
@@ -122,3 +123,70 @@ class EmulDomain:
         for n, a in zip(names, nxt):
             if n in primary:
                 self.arrays[n][...] = a
+
+
+# ---- discrete adjoint (capaz_adjoint.h) on the host -----------------------------
+
+ADJ_STATE = ("P_w", "S_w", "S_n", "P_n", "P_g")
+
+
+def adjoint_step(keys: dict, state: dict, lam1: dict, grad: dict) -> dict:
+    """One adjoint step: ``state`` holds x_t (P_w, S_w, S_n, P_n, P_g) and the
+    parameters m, K; ``lam1`` the adjoint of x_{t+1}.  Returns lambda_t and
+    adds the step's share to grad["lnK"], grad["m"]."""
+    p = params_from_keys(keys)
+    shape = state["P_w"].shape
+    dtm = p.dt / state["m"]
+    st = [np.ascontiguousarray(state[n]) for n in ADJ_STATE] + [np.ascontiguousarray(state["m"]),
+                                                              np.ascontiguousarray(state["K"]), dtm]
+    l1 = [np.ascontiguousarray(lam1[n]) for n in ADJ_STATE]
+    work = [np.zeros(shape) for _ in range(6)]
+    lam = {n: np.zeros(shape) for n in ADJ_STATE}
+    P = ctypes.POINTER(ctypes.c_double)
+    tab = lambda arrs: (P * len(arrs))(*[_ptr(a) for a in arrs])
+    rc = lib().capaz_emul_adjoint_step(ctypes.byref(p), tab(st), tab(l1), tab(work), tab([lam[n] for n in ADJ_STATE]),
+                                       tab([grad["lnK"], grad["m"]]))
+    if rc:
+        raise RuntimeError("emul adjoint step failed")
+    return lam
+
+
+def objective(states_at: dict, obs: dict) -> float:
+    """0.5 * sum over the observed steps and cells of the squared S_w and
+    S_n mismatch; states_at[step] = (S_w, S_n) flattened."""
+    J = 0.0
+    for s_i, step in enumerate(obs["steps"]):
+        Sw, Sn = states_at[step]
+        J += 0.5 * float(np.sum((Sw[obs["cells"]] - obs["S_w"][s_i]) ** 2 + (Sn[obs["cells"]] - obs["S_n"][s_i]) ** 2))
+    return J
+
+
+def gradient(keys: dict, fields: dict, nsteps: int, obs: dict):
+    """J and dJ/d lnK, dJ/dm of `nsteps` fused steps from `fields`, by the
+    host build of the forward step and of the adjoint (all states kept)."""
+    d = EmulDomain(keys, fields)
+    states, seen = [], {}
+    for t in range(nsteps):
+        states.append({n: d.arrays[n].copy() for n in ADJ_STATE + ("m", "K")})
+        d.fused_step()
+        if t + 1 in obs["steps"]:
+            seen[t + 1] = (d.arrays["S_w"].ravel().copy(), d.arrays["S_n"].ravel().copy())
+    J = objective(seen, obs)
+    shape = d.shape
+    grad = {"lnK": np.zeros(shape), "m": np.zeros(shape)}
+
+    def add_obs(lam, step):
+        if step in obs["steps"]:
+            s_i = list(obs["steps"]).index(step)
+            Sw, Sn = seen[step]
+            lw, ln = lam["S_w"].reshape(-1), lam["S_n"].reshape(-1)
+            np.add.at(lw, obs["cells"], Sw[obs["cells"]] - obs["S_w"][s_i])
+            np.add.at(ln, obs["cells"], Sn[obs["cells"]] - obs["S_n"][s_i])
+
+    lam = {n: np.zeros(shape) for n in ADJ_STATE}
+    add_obs(lam, nsteps)
+    for t in range(nsteps - 1, -1, -1):
+        lam = adjoint_step(keys, states[t], lam, grad)
+        if t >= 1:
+            add_obs(lam, t)
+    return J, grad["lnK"], grad["m"]

@@ -1,0 +1,130 @@
+"""The discrete adjoint through the C ABI on device 0 (capaz_objective,
+capaz_adjoint).
+
+  * J from capaz_objective and from capaz_adjoint equal the host build's J
+    bit for bit (the forward steps are the oracle's bits in the Corey
+    closure), and the gradient matches the host build of the same adjoint
+    bodies to rounding;
+  * the gradient does not depend on the checkpoint interval (each segment
+    is recomputed with the same bits);
+  * directional derivatives match central differences of capaz_objective;
+  * afterwards the context holds the state after nsteps, as capaz_step.
+"""
+import numpy as np
+import pytest
+
+from capaz_amd import capaz, workloads
+from capaz_cases import bit_equal, case, random_state
+from emul import emul
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("P_w", "S_w", "S_n", "P_n", "P_g")
+
+
+def problem(nx, ny, nz, seed, nsteps):
+    keys = case("3ph", nx, ny, nz, CLOSURE="corey", S_WR_0=0.1, S_NR_0=0.05, SOURCE=1, DT=1e-4, ITERATIONS=4)
+    st = random_state(keys, seed)
+    rng = np.random.default_rng(seed + 1)
+    shape = st["P_w"].shape
+    lnK = np.log(keys["K_0"]) + 0.5 * rng.standard_normal(shape)
+    st["K"] = np.exp(lnK)
+    cells = rng.choice(int(np.prod(shape)), 20, replace=False)
+    steps = sorted({max(1, nsteps // 3), max(1, 2 * nsteps // 3), nsteps})
+    obs = {"cells": cells, "steps": steps, "S_w": rng.uniform(0.2, 0.5, (len(steps), 20)),
+           "S_n": rng.uniform(0.1, 0.4, (len(steps), 20))}
+    return keys, st, lnK, obs, rng
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(a).max(), np.abs(b).max(), 1e-300))
+
+
+def test_adjoint_matches_host_build_and_checkpointing_is_exact():
+    keys, st, lnK, obs, rng = problem(20, 17, 12, 31, 9)
+    J_h, gK_h, gm_h = emul.gradient(keys, st, 9, obs)
+    d = capaz.Domain(keys, st)
+    assert d.objective(9, obs) == J_h
+    grads = []
+    for every in (0, 1, 2, 4, 9):
+        for f, v in st.items():
+            d.set_field(f, v)
+        J, gK, gm = d.adjoint(9, obs, checkpoint_every=every)
+        assert J == J_h
+        assert rel(gK, gK_h) < 1e-10 and rel(gm, gm_h) < 1e-10, f"checkpoint_every={every}"
+        grads.append((gK, gm))
+    for gK, gm in grads[1:]:
+        assert bit_equal(gK, grads[0][0]) and bit_equal(gm, grads[0][1])
+    # the context holds the state after 9 steps
+    ref = capaz.Domain(keys, st)
+    ref.run(9)
+    for f in ("P_w", "S_w", "S_n"):
+        assert bit_equal(d.field(f), ref.field(f)), f
+    d.close()
+    ref.close()
+
+
+def test_adjoint_directional_derivatives_match_central_differences():
+    keys, st, lnK, obs, rng = problem(24, 20, 16, 41, 12)
+    d = capaz.Domain(keys, st)
+    J, gK, gm = d.adjoint(12, obs)
+
+    def J_at(lnK_, m_):
+        for f, v in st.items():
+            d.set_field(f, v)
+        d.set_field("K", np.exp(lnK_))
+        d.set_field("m", m_)
+        return d.objective(12, obs)
+
+    eps = 1e-4
+    for _ in range(4):
+        v = rng.standard_normal(lnK.shape)
+        fd = (J_at(lnK + eps * v, st["m"]) - J_at(lnK - eps * v, st["m"])) / (2 * eps)
+        ad = float(np.sum(gK * v))
+        assert abs(fd - ad) <= 1e-5 * abs(fd), (fd, ad)
+        vm = 0.05 * rng.standard_normal(lnK.shape)
+        fd = (J_at(lnK, st["m"] + eps * vm) - J_at(lnK, st["m"] - eps * vm)) / (2 * eps)
+        ad = float(np.sum(gm * vm))
+        assert abs(fd - ad) <= 1e-4 * abs(fd), (fd, ad)
+    d.close()
+
+
+def test_history_matching_workload_small():
+    keys, st, lnK_true, lnK0, ob = workloads.history_matching(n=32, nsteps=40, columns=4, every=10)
+    t = capaz.Domain(keys, dict(st, K=np.exp(lnK_true)))
+    seen_w, seen_n = [], []
+    for s in range(1, 41):
+        t.run(1, first_step=s)
+        if s in ob["steps"]:
+            seen_w.append(t.field("S_w").ravel()[ob["cells"]])
+            seen_n.append(t.field("S_n").ravel()[ob["cells"]])
+    t.close()
+    obs = dict(ob, S_w=np.array(seen_w), S_n=np.array(seen_n))
+    d = capaz.Domain(keys, dict(st, K=np.exp(lnK0)))
+    J, gK, gm = d.adjoint(40, obs)
+    assert J > 0 and np.isfinite(gK).all() and np.isfinite(gm).all() and np.abs(gK).max() > 0
+    d.close()
+
+
+@pytest.mark.parametrize("over,msg", [(dict(MODEL=2), "three-phase"), (dict(CLOSURE="reference"), "Corey"),
+                                      (dict(ITERATIONS=9), "8 Newton")])
+def test_adjoint_rejects_unsupported(over, msg):
+    keys, st, lnK, obs, rng = problem(8, 8, 8, 51, 3)
+    keys.update(over)
+    if over.get("MODEL") == 2:
+        st = {k: v for k, v in st.items() if k in ("P_w", "S_n", "m", "K")}
+    d = capaz.Domain(keys, st)
+    with pytest.raises(capaz.CapazError, match=msg):
+        d.adjoint(3, obs)
+    d.close()
+
+
+def test_adjoint_rejects_bad_observations():
+    keys, st, lnK, obs, rng = problem(8, 8, 8, 52, 3)
+    d = capaz.Domain(keys, st)
+    bad = dict(obs, cells=np.array([1, 1] + list(obs["cells"][2:])))
+    with pytest.raises(capaz.CapazError, match="repeated"):
+        d.adjoint(3, bad)
+    with pytest.raises(capaz.CapazError, match="increase"):
+        d.objective(3, dict(obs, steps=[2, 1, 3]))
+    d.close()
