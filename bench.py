@@ -3,7 +3,7 @@
 filtration step at 256^3 on one MI355X, with the HBM roofline of the fused
 step kernel and the CPU oracle timed beside it.
 
-    python bench.py [--steps K] [--warmup W] [--n 256] [--no-cpu-baseline] [--no-ensemble]
+    python bench.py [--steps K] [--warmup W] [--n 256] [--no-cpu-baseline] [--no-ensemble] [--no-adjoint]
 
 A step is one fused time step (capaz_step: closures, upwind Darcy fluxes,
 mass update, 4-iteration 3x3 Newton per cell and the boundary faces) over all
@@ -131,6 +131,76 @@ def bench_ensemble(steps: int = 500, count: int = 64) -> dict:
             "speedup": round(wall_s / wall_b, 2)}
 
 
+def bench_adjoint(n: int = 96, nsteps: int = 400, directions: int = 20) -> dict:
+    """The permeability/porosity inversion workload (capaz_amd.workloads.
+    history_matching: three-phase n^3, nsteps steps, 16 observation columns
+    sampled every 50 steps, observations from a seeded true ln K, trial ln K
+    at its mean): wall time of the objective alone (capaz_objective) and of
+    objective + gradient (capaz_adjoint, sqrt(N) checkpoints), and central
+    differences of capaz_objective along random directions in (ln K, m)."""
+    from capaz_amd import capaz, workloads
+
+    keys, st, lnK_true, lnK0, ob = workloads.history_matching(n, nsteps)
+    t = capaz.Domain(keys, dict(st, K=np.exp(lnK_true)))
+    seen_w, seen_n = [], []
+    done = 0
+    for s in ob["steps"]:
+        t.run(int(s) - done, first_step=done + 1)
+        done = int(s)
+        seen_w.append(t.field("S_w").ravel()[ob["cells"]])
+        seen_n.append(t.field("S_n").ravel()[ob["cells"]])
+    t.close()
+    obs = dict(ob, S_w=np.array(seen_w), S_n=np.array(seen_n))
+    m0 = st["m"]
+    d = capaz.Domain(keys, dict(st, K=np.exp(lnK0)))
+
+    def reset(lnK, m):
+        for f in ("P_w", "S_w", "S_n", "P_n", "P_g"):
+            d.set_field(f, st[f])
+        d.set_field("K", np.exp(lnK))
+        d.set_field("m", m)
+
+    def timed(fn, reps=3):
+        best = None
+        for _ in range(reps):
+            reset(lnK0, m0)
+            t0 = time.perf_counter()
+            out = fn()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return out, best
+
+    J, t_fwd = timed(lambda: d.objective(nsteps, obs))
+    (J_a, gK, gm), t_adj = timed(lambda: d.adjoint(nsteps, obs))
+    (_, gK1, gm1), t_adj1 = timed(lambda: d.adjoint(nsteps, obs, checkpoint_every=1))
+    every = int(round(nsteps ** 0.5))
+    rng = np.random.default_rng(2024)
+    errs = {1e-6: [], 1e-5: []}
+    for _ in range(directions):
+        vK = rng.standard_normal(lnK0.shape)
+        vm = 0.02 * rng.standard_normal(lnK0.shape)
+        ad = float(np.sum(gK * vK) + np.sum(gm * vm))
+        for eps in errs:
+            reset(lnK0 + eps * vK, m0 + eps * vm)
+            jp = d.objective(nsteps, obs)
+            reset(lnK0 - eps * vK, m0 - eps * vm)
+            jm = d.objective(nsteps, obs)
+            fd = (jp - jm) / (2 * eps)
+            errs[eps].append(abs(fd - ad) / max(abs(fd), 1e-300))
+    d.close()
+    return {"workload": f"three-phase {n}^3, {nsteps} steps, 16 observation columns every 50 steps, "
+                        f"gradient in per-cell ln K and porosity",
+            "J": J, "J_equal": J == J_a,
+            "forward_ms": round(t_fwd * 1e3, 2), "adjoint_ms": round(t_adj * 1e3, 2),
+            "adjoint_over_forward": round(t_adj / t_fwd, 2), "checkpoint_every": every,
+            "adjoint_store_all_ms": round(t_adj1 * 1e3, 2),
+            "adjoint_store_all_over_forward": round(t_adj1 / t_fwd, 2),
+            "checkpointing_changes_gradient": bool(not (np.array_equal(gK, gK1) and np.array_equal(gm, gm1))),
+            "fd": {"directions": directions, "direction": "ln K ~ N(0,1), m ~ 0.02 N(0,1) per cell",
+                   **{f"max_rel_err_eps_{eps:g}": float(max(v)) for eps, v in errs.items()},
+                   **{f"median_rel_err_eps_{eps:g}": float(np.median(v)) for eps, v in errs.items()}}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +209,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a copy of known size, to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -225,6 +296,9 @@ def main() -> int:
     if not args.no_ensemble:
         print("bench: batched ensemble", file=sys.stderr, flush=True)
         out["ensemble"] = bench_ensemble()
+    if not args.no_adjoint:
+        print("bench: adjoint", file=sys.stderr, flush=True)
+        out["adjoint"] = bench_adjoint()
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(args.n)

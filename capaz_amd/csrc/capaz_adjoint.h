@@ -27,8 +27,8 @@
 //     neighbours that read its coefficients -- and chains the coefficients'
 //     adjoint to the state (P_a, S_w, S_n) and to ln K.
 //
-// The upwind switches are differentiated where they are smooth (the kink
-// at a zero driving force has measure zero).  Nothing here needs to match
+// The upwind switches are differentiated piecewise, with the mean of the
+// one-sided slopes at a zero driving force.  Nothing here needs to match
 // the forward step bit for bit; the adjoint is checked against central
 // finite differences of the forward step.
 #pragma once
@@ -298,13 +298,20 @@ CAPAZ_HD void adj_newton_cell(const DevParams& p, const AdjIO& a, int i, int j, 
 // Face-term derivatives of directed() for one cell's flux along one axis.
 // x1, x2: the cell's minus / plus driving forces; L = -Xi*ro of the cell (c)
 // and of its minus (m) / plus (p) neighbours; rh: 1 / the axis spacing.
+// At a zero driving force the upwind choice switches: there the derivative
+// of max(x, 0) is taken as 1/2 (the mean of the two one-sided slopes, which
+// is what a central difference measures; states that are uniform along an
+// axis put many faces exactly there).
+CAPAZ_HD double upwind_w(double x) { return x > 0. ? 1. : (x < 0. ? 0. : 0.5); }
+
 struct DirD {
     double dx1, dx2, dLc, dLm, dLp;
 };
 CAPAZ_HD DirD directed_d(double x1, double x2, double Lc, double Lm, double Lp, double rh) {
     DirD d;
-    d.dx1 = (-(x1 < 0. ? Lc : 0.) - (x1 > 0. ? Lm : 0.)) * rh;
-    d.dx2 = ((x2 > 0. ? Lc : 0.) + (x2 < 0. ? Lp : 0.)) * rh;
+    const double w1 = upwind_w(x1), w2 = upwind_w(x2);
+    d.dx1 = (-(1. - w1) * Lc - w1 * Lm) * rh;
+    d.dx2 = (w2 * Lc + (1. - w2) * Lp) * rh;
     d.dLc = ((x2 > 0. ? x2 : 0.) - (x1 < 0. ? x1 : 0.)) * rh;
     d.dLm = -(x1 > 0. ? x1 : 0.) * rh;
     d.dLp = (x2 < 0. ? x2 : 0.) * rh;
@@ -360,7 +367,8 @@ CAPAZ_HD void adj_coef_cell(const DevParams& p, const AdjIO& a, int i, int j, in
                 const double gr = grav ? g * cp.ro[ph] : 0.;
                 const double x1 = -((cp.P[ph] - c.P[ph]) * rh) + gr;
                 const double Lq = -cp.Xi[ph] * cp.ro[ph];
-                const double dx1 = (-(x1 < 0. ? Lq : 0.) - (x1 > 0. ? Lc : 0.)) * rh;
+                const double w1 = upwind_w(x1);
+                const double dx1 = (-(1. - w1) * Lq - w1 * Lc) * rh;
                 lP[ph] += mu * dx1 * rh;  // dx1/dP_m = +1/h
                 lL[ph] += mu * (-(x1 > 0. ? x1 : 0.) * rh);
             }
@@ -369,7 +377,8 @@ CAPAZ_HD void adj_coef_cell(const DevParams& p, const AdjIO& a, int i, int j, in
                 const double gr = grav ? g * cm.ro[ph] : 0.;
                 const double x2 = -((c.P[ph] - cm.P[ph]) * rh) + gr;
                 const double Lq = -cm.Xi[ph] * cm.ro[ph];
-                const double dx2 = ((x2 > 0. ? Lq : 0.) + (x2 < 0. ? Lc : 0.)) * rh;
+                const double w2 = upwind_w(x2);
+                const double dx2 = (w2 * Lq + (1. - w2) * Lc) * rh;
                 lP[ph] += mu * (-dx2 * rh);  // dx2/dP_p = -1/h
                 lL[ph] += mu * ((x2 < 0. ? x2 : 0.) * rh);
             }

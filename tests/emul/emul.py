@@ -153,12 +153,16 @@ def adjoint_step(keys: dict, state: dict, lam1: dict, grad: dict) -> dict:
 
 def objective(states_at: dict, obs: dict) -> float:
     """0.5 * sum over the observed steps and cells of the squared S_w and
-    S_n mismatch; states_at[step] = (S_w, S_n) flattened."""
-    J = 0.0
+    S_n mismatch; states_at[step] = (S_w, S_n) flattened.  Summed sample by
+    sample, cell by cell, as capaz_objective does (the same bits)."""
+    total = 0.0
     for s_i, step in enumerate(obs["steps"]):
         Sw, Sn = states_at[step]
-        J += 0.5 * float(np.sum((Sw[obs["cells"]] - obs["S_w"][s_i]) ** 2 + (Sn[obs["cells"]] - obs["S_n"][s_i]) ** 2))
-    return J
+        dw = Sw[obs["cells"]] - obs["S_w"][s_i]
+        dn = Sn[obs["cells"]] - obs["S_n"][s_i]
+        for a, b in zip(dw.tolist(), dn.tolist()):
+            total += a * a + b * b
+    return 0.5 * total
 
 
 def gradient(keys: dict, fields: dict, nsteps: int, obs: dict):

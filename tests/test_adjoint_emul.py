@@ -77,3 +77,39 @@ def test_gradient_vanishes_where_nothing_is_observed_downstream():
     assert len(nz) > 0
     assert np.abs(nz - np.array([3, 3, 3])).max() <= 1  # the 7-point stencil of the observed cell
     assert np.count_nonzero(g_m) == 1 and g_m[3, 3, 3] != 0
+
+
+def test_adjoint_at_zero_driving_forces_matches_central_differences():
+    # The inversion workload starts from a uniform pressure with a uniform
+    # trial K, so many x/z faces carry exactly zero driving force, where the
+    # upwind flux has a kink: the adjoint takes the mean of the one-sided
+    # slopes there, which is what a central difference measures.
+    from capaz_amd import workloads
+
+    n, N = 16, 24
+    keys, st, lnK_true, lnK0, ob = workloads.history_matching(n=n, nsteps=N, columns=4, every=N // 4)
+    steps = [int(s) for s in ob["steps"]]
+
+    def run(lnK, m):
+        d = oracle.OracleDomain(keys, dict(st, K=np.exp(lnK), m=m))
+        seen = {}
+        for t in range(1, N + 1):
+            d.run(1, first_step=t)
+            if t in steps:
+                seen[t] = (d.field("S_w").ravel().copy(), d.field("S_n").ravel().copy())
+        return seen
+
+    truth = run(lnK_true, st["m"])
+    obs = {"cells": ob["cells"], "steps": steps, "S_w": np.array([truth[s][0][ob["cells"]] for s in steps]),
+           "S_n": np.array([truth[s][1][ob["cells"]] for s in steps])}
+    J, gK, gm = emul.gradient(keys, dict(st, K=np.exp(lnK0)), N, obs)
+    assert J == emul.objective(run(lnK0, st["m"]), obs)
+    rng = np.random.default_rng(77)
+    eps = 1e-4
+    for _ in range(2):
+        vK = rng.standard_normal(lnK0.shape)
+        vm = 0.02 * rng.standard_normal(lnK0.shape)
+        fd = (emul.objective(run(lnK0 + eps * vK, st["m"] + eps * vm), obs)
+              - emul.objective(run(lnK0 - eps * vK, st["m"] - eps * vm), obs)) / (2 * eps)
+        ad = float(np.sum(gK * vK) + np.sum(gm * vm))
+        assert abs(fd - ad) <= 1e-5 * abs(fd), (fd, ad)
