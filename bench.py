@@ -173,6 +173,9 @@ def bench_adjoint(n: int = 96, nsteps: int = 400, directions: int = 20) -> dict:
     J, t_fwd = timed(lambda: d.objective(nsteps, obs))
     (J_a, gK, gm), t_adj = timed(lambda: d.adjoint(nsteps, obs))
     (_, gK1, gm1), t_adj1 = timed(lambda: d.adjoint(nsteps, obs, checkpoint_every=1))
+    d.tune(adjoint_kernel=0)
+    _, t_adj_cell = timed(lambda: d.adjoint(nsteps, obs))
+    d.tune(adjoint_kernel=1)
     every = int(round(nsteps ** 0.5))
     rng = np.random.default_rng(2024)
     errs = {1e-6: [], 1e-5: []}
@@ -194,6 +197,7 @@ def bench_adjoint(n: int = 96, nsteps: int = 400, directions: int = 20) -> dict:
             "forward_ms": round(t_fwd * 1e3, 2), "adjoint_ms": round(t_adj * 1e3, 2),
             "adjoint_over_forward": round(t_adj / t_fwd, 2), "checkpoint_every": every,
             "adjoint_store_all_ms": round(t_adj1 * 1e3, 2),
+            "adjoint_one_thread_per_cell_ms": round(t_adj_cell * 1e3, 2),
             "adjoint_store_all_over_forward": round(t_adj1 / t_fwd, 2),
             "checkpointing_changes_gradient": bool(not (np.array_equal(gK, gK1) and np.array_equal(gm, gm1))),
             "fd": {"directions": directions, "direction": "ln K ~ N(0,1), m ~ 0.02 N(0,1) per cell",
@@ -210,6 +214,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--only", choices=("ensemble", "adjoint"), default=None,
+                    help="run only this secondary measurement and print it")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a copy of known size, to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters), then exit")
@@ -227,6 +233,11 @@ def main() -> int:
         print(f"bench.py: this backend runs one domain on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU slab decomposition is a design, not a build)", file=sys.stderr)
         return 2
+    if args.only:
+        # one secondary measurement alone (for profiling), as its own JSON line
+        fn = {"ensemble": bench_ensemble, "adjoint": bench_adjoint}[args.only]
+        print(json.dumps({args.only: fn()}), flush=True)
+        return 0
 
     from capaz_amd import capaz, hip, workloads
 

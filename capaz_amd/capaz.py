@@ -56,6 +56,8 @@ EXPORTS = {
     "capaz_stream": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_void_p)]),
     "capaz_load_face": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "capaz_save_face": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "capaz_save_state": (ctypes.c_int, ["ctx", ctypes.c_char_p, ctypes.c_long]),
+    "capaz_load_state": (ctypes.c_int, ["ctx", ctypes.c_char_p, ctypes.POINTER(ctypes.c_long)]),
     "capaz_objective": (ctypes.c_int, ["ctx", ctypes.c_long, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "capaz_adjoint": (ctypes.c_int, ["ctx", ctypes.c_long, ctypes.c_void_p, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_double), "dbl_p", "dbl_p"]),
@@ -254,11 +256,21 @@ class Domain:
 
     def tuning(self) -> dict:
         out = {}
-        for k in ("fused_kernel", "tile_y", "kchunk", "timing"):
+        for k in ("fused_kernel", "tile_y", "kchunk", "adjoint_kernel", "timing"):
             v = ctypes.c_int()
             self._check(load().capaz_get_tuning(self._ctx, k.encode(), ctypes.byref(v)), "capaz_get_tuning")
             out[k] = v.value
         return out
+
+    def save_state(self, path: str, step: int = 0) -> None:
+        """capaz_save_state: the fields the next step reads, to `path`."""
+        self._check(load().capaz_save_state(self._ctx, os.fsencode(path), step), "capaz_save_state")
+
+    def load_state(self, path: str) -> int:
+        """capaz_load_state: restores a saved state; returns its step number."""
+        st = ctypes.c_long()
+        self._check(load().capaz_load_state(self._ctx, os.fsencode(path), ctypes.byref(st)), "capaz_load_state")
+        return st.value
 
     def objective(self, nsteps: int, obs: dict) -> float:
         """capaz_objective: J after nsteps fused steps from the current state

@@ -236,9 +236,12 @@ CAPAZ_HD void for_boundary_of(const DevParams& p, int i, int j, int k, Fn&& f) {
             }
 }
 
-// Pass 1 at interior cell (i, j, k), l its index (IT: newton3_adjoint's).
+// Pass 1 at interior cell (i, j, k), l its index, once the step's
+// densities ro, state (Pw, Sw, Sn) and flux divergence div of the cell are
+// known (IT: newton3_adjoint's).
 template <int IT>
-CAPAZ_HD void adj_newton_cell(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l) {
+CAPAZ_HD void adj_newton_tail(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l, const double* ro,
+                              double Pw, double Sw, double Sn, const double* div) {
     // lambda of x_{t+1}(l), plus the boundary cells copied from it
     double lam[3] = {a.lPw1[l], a.lSw1[l], a.lSn1[l]};
     for_boundary_of(p, i, j, k, [&](int bi, int bj, int bk) {
@@ -259,7 +262,28 @@ CAPAZ_HD void adj_newton_cell(const DevParams& p, const AdjIO& a, int i, int j, 
             lam[2] += a.lSn1[lb];
         }
     });
-    // forward quantities of the step at l
+    const double Sg = 1. - Sw - Sn, dtm = a.dtm[l];
+    double A[3];
+    A[0] = ro[0] * Sw - dtm * div[0];
+    A[1] = ro[1] * Sn - dtm * div[1];
+    A[2] = ro[2] * Sg - dtm * div[2];
+    double lA[3] = {0., 0., 0.};
+    newton3_adjoint<IT>(p, Pw, Sw, Sn, A, lam, lA);
+    a.mu_w[l] = -dtm * lA[0];
+    a.mu_n[l] = -dtm * lA[1];
+    a.mu_g[l] = -dtm * lA[2];
+    // dA/dm = div * dtm / m
+    a.g_m[l] += (lA[0] * div[0] + lA[1] * div[1] + lA[2] * div[2]) * dtm / a.m[l];
+    // roS_a(x_t(l)) and the Newton initial guess
+    const double rg = p.ro0_g * p.r_P_atm;
+    a.dPw[l] = lam[0] + lA[0] * p.c_wb * Sw + lA[1] * p.c_nb * Sn + lA[2] * rg * Sg;
+    a.dSw[l] = lam[1] + lA[0] * ro[0] - lA[2] * ro[2];
+    a.dSn[l] = lam[2] + lA[1] * ro[1] - lA[2] * ro[2];
+}
+
+// Pass 1 at interior cell (i, j, k), reading its neighbours from memory.
+template <int IT>
+CAPAZ_HD void adj_newton_cell(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l) {
     const Coef c = coef_v(p, a, l, true);
     auto nb = [&](int ii, int jj, int kk) {
         const int64_t q = (int64_t)ii + (int64_t)jj * p.sy + (int64_t)kk * p.sz;
@@ -275,24 +299,9 @@ CAPAZ_HD void adj_newton_cell(const DevParams& p, const AdjIO& a, int i, int j, 
         zp = nb(i, j, k + 1);
     }
     const Coef ym = nb(i, j - 1, k), yp = nb(i, j + 1, k);
-    const double Sw = a.Sw[l], Sn = a.Sn[l], Sg = 1. - Sw - Sn, dtm = a.dtm[l];
-    double div[3], A[3];
+    double div[3];
     for (int ph = 0; ph < 3; ++ph) div[ph] = coef_divergence(p, ph, c, xm, xp, ym, yp, zm, zp);
-    A[0] = c.ro[0] * Sw - dtm * div[0];
-    A[1] = c.ro[1] * Sn - dtm * div[1];
-    A[2] = c.ro[2] * Sg - dtm * div[2];
-    double lA[3] = {0., 0., 0.};
-    newton3_adjoint<IT>(p, a.Pw[l], Sw, Sn, A, lam, lA);
-    a.mu_w[l] = -dtm * lA[0];
-    a.mu_n[l] = -dtm * lA[1];
-    a.mu_g[l] = -dtm * lA[2];
-    // dA/dm = div * dtm / m
-    a.g_m[l] += (lA[0] * div[0] + lA[1] * div[1] + lA[2] * div[2]) * dtm / a.m[l];
-    // roS_a(x_t(l)) and the Newton initial guess
-    const double rg = p.ro0_g * p.r_P_atm;
-    a.dPw[l] = lam[0] + lA[0] * p.c_wb * Sw + lA[1] * p.c_nb * Sn + lA[2] * rg * Sg;
-    a.dSw[l] = lam[1] + lA[0] * c.ro[0] - lA[2] * c.ro[2];
-    a.dSn[l] = lam[2] + lA[1] * c.ro[1] - lA[2] * c.ro[2];
+    adj_newton_tail<IT>(p, a, i, j, k, l, c.ro, a.Pw[l], a.Sw[l], a.Sn[l], div);
 }
 
 // Face-term derivatives of directed() for one cell's flux along one axis.
@@ -318,73 +327,58 @@ CAPAZ_HD DirD directed_d(double x1, double x2, double Lc, double Lm, double Lp, 
     return d;
 }
 
-// Pass 2 at cell (i, j, k), l its index: lambda_t(l) and the ln K gradient.
-CAPAZ_HD void adj_coef_cell(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l) {
-    const bool inner = interior3(p, i, j, k);
-    const CoefD c = coef_d(p, a, l, inner);
-    double lP[3] = {0., 0., 0.}, lL[3] = {0., 0., 0.}, lro[3] = {0., 0., 0.};
-    const double g = p.g;
-    // axis: stride, spacing, active, gravity
-    for (int ax = 0; ax < 3; ++ax) {
-        int di = 0, dj = 0, dk = 0, n = 0, pos = 0;
-        double rh;
-        bool grav = false;
-        if (ax == 0) {
-            if (p.Nx < 2) continue;
-            di = 1; n = p.Nx; pos = i; rh = p.r_hx;
-        } else if (ax == 1) {
-            dj = 1; n = p.Ny; pos = j; rh = p.r_hy; grav = true;
-        } else {
-            if (p.Nz < 2) continue;
-            dk = 1; n = p.Nz; pos = k; rh = p.r_hz;
+// A neighbour along one axis as pass 2 reads it: its coefficients, its mu
+// (zero unless it is an interior cell) and whether it exists.
+struct AdjNb {
+    Coef c;
+    double mu[3];
+    bool exists, interior;
+};
+
+// Pass 2, one axis of cell c: adds to lP, lL (L = -Xi ro) and lro the
+// derivative of the cell's own flux (weighted by its mu, if interior) and of
+// its interior neighbours' fluxes that read the cell.
+CAPAZ_HD void adj_coef_axis(const DevParams& p, const CoefD& c, const double* mu_self, const AdjNb& m,
+                            const AdjNb& q, double rh, bool grav, double* lP, double* lL, double* lro) {
+#if defined(__HIPCC__)
+#pragma unroll
+#endif
+    for (int ph = 0; ph < 3; ++ph) {
+        const double Lc = -c.Xi[ph] * c.ro[ph];
+        if (c.inner) {  // the cell's own flux along this axis (both neighbours exist)
+            const double gr = grav ? p.g * c.ro[ph] : 0.;
+            const double x1 = -((c.P[ph] - m.c.P[ph]) * rh) + gr, x2 = -((q.c.P[ph] - c.P[ph]) * rh) + gr;
+            const DirD d = directed_d(x1, x2, Lc, -m.c.Xi[ph] * m.c.ro[ph], -q.c.Xi[ph] * q.c.ro[ph], rh);
+            lP[ph] += mu_self[ph] * (d.dx2 - d.dx1) * rh;  // dx1/dP_c = -1/h, dx2/dP_c = +1/h
+            lL[ph] += mu_self[ph] * d.dLc;
+            if (grav) lro[ph] += mu_self[ph] * (d.dx1 + d.dx2) * p.g;
         }
-        const bool has_m = pos - 1 >= 0, has_p = pos + 1 < n;
-        Coef cm{}, cp{};
-        bool m_in = false, p_in = false;
-        if (has_m) {
-            m_in = interior3(p, i - di, j - dj, k - dk);
-            cm = coef_v(p, a, l - (int64_t)di - (int64_t)dj * p.sy - (int64_t)dk * p.sz, m_in);
+        if (q.exists && q.interior) {  // plus neighbour's flux: this cell is its minus neighbour
+            const double gr = grav ? p.g * q.c.ro[ph] : 0.;
+            const double x1 = -((q.c.P[ph] - c.P[ph]) * rh) + gr;
+            const double Lq = -q.c.Xi[ph] * q.c.ro[ph];
+            const double w1 = upwind_w(x1);
+            const double dx1 = (-(1. - w1) * Lq - w1 * Lc) * rh;
+            lP[ph] += q.mu[ph] * dx1 * rh;  // dx1/dP_m = +1/h
+            lL[ph] += q.mu[ph] * (-(x1 > 0. ? x1 : 0.) * rh);
         }
-        if (has_p) {
-            p_in = interior3(p, i + di, j + dj, k + dk);
-            cp = coef_v(p, a, l + (int64_t)di + (int64_t)dj * p.sy + (int64_t)dk * p.sz, p_in);
-        }
-        const int64_t qm = l - (int64_t)di - (int64_t)dj * p.sy - (int64_t)dk * p.sz;
-        const int64_t qp = l + (int64_t)di + (int64_t)dj * p.sy + (int64_t)dk * p.sz;
-        for (int ph = 0; ph < 3; ++ph) {
-            const double mu_self = inner ? (ph == 0 ? a.mu_w[l] : ph == 1 ? a.mu_n[l] : a.mu_g[l]) : 0.;
-            const double Lc = -c.Xi[ph] * c.ro[ph];
-            if (inner) {  // the cell's own flux along this axis (both neighbours exist)
-                const double gr = grav ? g * c.ro[ph] : 0.;
-                const double x1 = -((c.P[ph] - cm.P[ph]) * rh) + gr, x2 = -((cp.P[ph] - c.P[ph]) * rh) + gr;
-                const DirD d = directed_d(x1, x2, Lc, -cm.Xi[ph] * cm.ro[ph], -cp.Xi[ph] * cp.ro[ph], rh);
-                lP[ph] += mu_self * (d.dx2 - d.dx1) * rh;  // dx1/dP_c = -1/h, dx2/dP_c = +1/h
-                lL[ph] += mu_self * d.dLc;
-                if (grav) lro[ph] += mu_self * (d.dx1 + d.dx2) * g;
-            }
-            if (has_p && p_in) {  // plus neighbour's flux: this cell is its minus neighbour
-                const double mu = ph == 0 ? a.mu_w[qp] : ph == 1 ? a.mu_n[qp] : a.mu_g[qp];
-                const double gr = grav ? g * cp.ro[ph] : 0.;
-                const double x1 = -((cp.P[ph] - c.P[ph]) * rh) + gr;
-                const double Lq = -cp.Xi[ph] * cp.ro[ph];
-                const double w1 = upwind_w(x1);
-                const double dx1 = (-(1. - w1) * Lq - w1 * Lc) * rh;
-                lP[ph] += mu * dx1 * rh;  // dx1/dP_m = +1/h
-                lL[ph] += mu * (-(x1 > 0. ? x1 : 0.) * rh);
-            }
-            if (has_m && m_in) {  // minus neighbour's flux: this cell is its plus neighbour
-                const double mu = ph == 0 ? a.mu_w[qm] : ph == 1 ? a.mu_n[qm] : a.mu_g[qm];
-                const double gr = grav ? g * cm.ro[ph] : 0.;
-                const double x2 = -((c.P[ph] - cm.P[ph]) * rh) + gr;
-                const double Lq = -cm.Xi[ph] * cm.ro[ph];
-                const double w2 = upwind_w(x2);
-                const double dx2 = (w2 * Lq + (1. - w2) * Lc) * rh;
-                lP[ph] += mu * (-dx2 * rh);  // dx2/dP_p = -1/h
-                lL[ph] += mu * ((x2 < 0. ? x2 : 0.) * rh);
-            }
+        if (m.exists && m.interior) {  // minus neighbour's flux: this cell is its plus neighbour
+            const double gr = grav ? p.g * m.c.ro[ph] : 0.;
+            const double x2 = -((c.P[ph] - m.c.P[ph]) * rh) + gr;
+            const double Lq = -m.c.Xi[ph] * m.c.ro[ph];
+            const double w2 = upwind_w(x2);
+            const double dx2 = (w2 * Lq + (1. - w2) * Lc) * rh;
+            lP[ph] += m.mu[ph] * (-dx2 * rh);  // dx2/dP_p = -1/h
+            lL[ph] += m.mu[ph] * ((x2 < 0. ? x2 : 0.) * rh);
         }
     }
-    // coefficients -> state and ln K
+}
+
+// Pass 2, after the axes: the coefficients' adjoint chained to the state and
+// ln K of cell l, plus pass 1's share if the cell is interior.
+CAPAZ_HD void adj_coef_finish(const DevParams& p, const AdjIO& a, int64_t l, const CoefD& c, const double* lP,
+                              const double* lL, const double* lro) {
+    (void)p;
     double lPw = 0., lSw = 0., lSn = 0., lPn = 0., lPg = 0., glnK = 0.;
     for (int ph = 0; ph < 3; ++ph) {
         const double lXi = -lL[ph] * c.ro[ph];
@@ -400,7 +394,7 @@ CAPAZ_HD void adj_coef_cell(const DevParams& p, const AdjIO& a, int i, int j, in
         lSn += lXi * c.dXi_dSn[ph];
         glnK += lXi * c.Xi[ph];  // dXi/dlnK = Xi
     }
-    if (inner) {
+    if (c.inner) {
         lPw += a.dPw[l];
         lSw += a.dSw[l];
         lSn += a.dSn[l];
@@ -411,6 +405,41 @@ CAPAZ_HD void adj_coef_cell(const DevParams& p, const AdjIO& a, int i, int j, in
     a.lPn[l] = lPn;
     a.lPg[l] = lPg;
     a.g_lnK[l] += glnK;
+}
+
+// Pass 2 at cell (i, j, k), l its index, reading its neighbours from memory.
+CAPAZ_HD AdjNb adj_nb_at(const DevParams& p, const AdjIO& a, int i, int j, int k) {
+    AdjNb n;
+    n.exists = i >= 0 && i < p.Nx && j >= 0 && j < p.Ny && k >= 0 && k < p.Nz;
+    n.interior = false;
+    n.mu[0] = n.mu[1] = n.mu[2] = 0.;
+    n.c = Coef{};
+    if (!n.exists) return n;
+    const int64_t q = (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz;
+    n.interior = interior3(p, i, j, k);
+    n.c = coef_v(p, a, q, n.interior);
+    if (n.interior) {
+        n.mu[0] = a.mu_w[q];
+        n.mu[1] = a.mu_n[q];
+        n.mu[2] = a.mu_g[q];
+    }
+    return n;
+}
+
+CAPAZ_HD void adj_coef_cell(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l) {
+    const bool inner = interior3(p, i, j, k);
+    const CoefD c = coef_d(p, a, l, inner);
+    const double mu_self[3] = {inner ? a.mu_w[l] : 0., inner ? a.mu_n[l] : 0., inner ? a.mu_g[l] : 0.};
+    double lP[3] = {0., 0., 0.}, lL[3] = {0., 0., 0.}, lro[3] = {0., 0., 0.};
+    if (!(p.Nx < 2))
+        adj_coef_axis(p, c, mu_self, adj_nb_at(p, a, i - 1, j, k), adj_nb_at(p, a, i + 1, j, k), p.r_hx, false, lP,
+                      lL, lro);
+    adj_coef_axis(p, c, mu_self, adj_nb_at(p, a, i, j - 1, k), adj_nb_at(p, a, i, j + 1, k), p.r_hy, true, lP, lL,
+                  lro);
+    if (!(p.Nz < 2))
+        adj_coef_axis(p, c, mu_self, adj_nb_at(p, a, i, j, k - 1), adj_nb_at(p, a, i, j, k + 1), p.r_hz, false, lP,
+                      lL, lro);
+    adj_coef_finish(p, a, l, c, lP, lL, lro);
 }
 
 }  // namespace capaz

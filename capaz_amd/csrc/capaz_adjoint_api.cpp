@@ -272,7 +272,9 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
             a.lPg = l0 + 4 * S;
             a.g_lnK = grad;
             a.g_m = grad + S;
-            if ((e = launch_adjoint_step(c->p, a, c->stream)) != hipSuccess)
+            e = c->tuning.adjoint_kernel ? launch_adjoint_step_tiled(c->p, a, c->stream)
+                                         : launch_adjoint_step(c->p, a, c->stream);
+            if (e != hipSuccess)
                 return hip_fail(c, e, "launch_adjoint_step");
             if (t >= 1 && (rc = obs_adjoint(l0, t))) return rc;
             cur ^= 1;

@@ -7,8 +7,10 @@
 
 namespace capaz {
 
-// Both passes of one adjoint step over the whole grid.
+// Both passes of one adjoint step over the whole grid: one thread per cell
+// reading neighbours from memory, or the z-marching LDS tiles.
 hipError_t launch_adjoint_step(const DevParams& p, const AdjIO& a, hipStream_t s);
+hipError_t launch_adjoint_step_tiled(const DevParams& p, const AdjIO& a, hipStream_t s);
 hipError_t launch_obs_gather(const double* Sw, const double* Sn, const int64_t* cells, int64_t ncells, double* out_w,
                              double* out_n, hipStream_t s);
 hipError_t launch_obs_adjoint(double* lSw, double* lSn, const int64_t* cells, int64_t ncells, const double* seen_w,

@@ -379,6 +379,8 @@ int capaz_tune(capaz_ctx* c, const char* key, int value) {
         c->tuning.tile_y = value;
     else if (k == "kchunk" && value >= 1 && value <= 1 << 20)
         c->tuning.kchunk = value;
+    else if (k == "adjoint_kernel" && (value == 0 || value == 1))
+        c->tuning.adjoint_kernel = value;
     else if (k == "timing" && (value == 0 || value == 1)) {
         int rc;
         if ((rc = check_ctx(c)) || (rc = drain_events(c))) return rc;
@@ -396,6 +398,7 @@ int capaz_get_tuning(const capaz_ctx* c, const char* key, int* value) {
     if (k == "fused_kernel") *value = c->tuning.fused_kernel;
     else if (k == "tile_y") *value = c->tuning.tile_y;
     else if (k == "kchunk") *value = c->tuning.kchunk;
+    else if (k == "adjoint_kernel") *value = c->tuning.adjoint_kernel;
     else if (k == "timing") *value = c->timing ? 1 : 0;
     else return CAPAZ_ERR_INVALID;
     return CAPAZ_OK;

@@ -151,9 +151,12 @@ struct FusedLds {
 // CARRY: keep the next plane's coefficients of the own column in registers
 // (computed for the z flux) as the next centre, or recompute the centre at
 // the top of every plane (fewer live registers across the Newton solve).
-template <int MODEL, int CLOSURE, int TX, int TY, bool CARRY>
+// Epi: what happens at an interior cell once its coefficients c and flux
+// divergence div are known -- the Newton solve and store of the step
+// (FusedStore) or the adjoint's backward Newton (AdjNewtonStore).
+template <int MODEL, int CLOSURE, int TX, int TY, bool CARRY, class Epi>
 __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& io, int kchunk, int bx, int by, int bz,
-                                             FusedLds<3 * MODEL, TX, TY>& lds) {
+                                             FusedLds<3 * MODEL, TX, TY>& lds, const Epi& epi) {
     static_assert(TY >= 3 && TX >= 2 * TY, "halo ring needs TY >= 3 and TX >= 2*TY");
     constexpr int NPH = MODEL;
 
@@ -245,23 +248,7 @@ __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& i
                 double s = (a == 2 ? 0. + fx : fx) + fy;
                 div[a] = s + fz;
             }
-            const double dtm = io.dtm[l];  // dt / m, computed once per porosity field
-            double Pw = c.P[0], Sn = c.Sn, Sw = c.Sw;
-            if constexpr (MODEL == 2) {
-                const double A1 = c.ro[0] * (1. - Sn) - dtm * div[0];
-                const double A2 = c.ro[1] * Sn - dtm * div[1];
-                newton2(p, Pw, Sn, A1, A2);
-                io.Pw1[l] = Pw;
-                io.Sn1[l] = Sn;
-            } else {
-                const double A1 = c.ro[0] * Sw - dtm * div[0];
-                const double A2 = c.ro[1] * Sn - dtm * div[1];
-                const double A3 = c.ro[2] * (1. - Sw - Sn) - dtm * div[2];
-                newton3(p, Pw, Sw, Sn, A1, A2, A3);
-                io.Pw1[l] = Pw;
-                io.Sw1[l] = Sw;
-                io.Sn1[l] = Sn;
-            }
+            epi(p, i, j, k, l, c, div);
         }
         __syncthreads();
         // roll the z window
@@ -275,10 +262,38 @@ __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& i
     }
 }
 
+// The step's epilogue: A = roS - dt/m * div, the Newton solve, the new state.
+template <int MODEL>
+struct FusedStore {
+    StepIO io;
+    __device__ __forceinline__ void operator()(const DevParams& p, int i, int j, int k, int64_t l,
+                                               const CoefT<MODEL>& c, const double* div) const {
+        (void)i; (void)j; (void)k;
+        const double dtm = io.dtm[l];  // dt / m, computed once per porosity field
+        double Pw = c.P[0], Sn = c.Sn, Sw = c.Sw;
+        if constexpr (MODEL == 2) {
+            const double A1 = c.ro[0] * (1. - Sn) - dtm * div[0];
+            const double A2 = c.ro[1] * Sn - dtm * div[1];
+            newton2(p, Pw, Sn, A1, A2);
+            io.Pw1[l] = Pw;
+            io.Sn1[l] = Sn;
+        } else {
+            const double A1 = c.ro[0] * Sw - dtm * div[0];
+            const double A2 = c.ro[1] * Sn - dtm * div[1];
+            const double A3 = c.ro[2] * (1. - Sw - Sn) - dtm * div[2];
+            newton3(p, Pw, Sw, Sn, A1, A2, A3);
+            io.Pw1[l] = Pw;
+            io.Sw1[l] = Sw;
+            io.Sn1[l] = Sn;
+        }
+    }
+};
+
 template <int MODEL, int CLOSURE, int TX, int TY, bool CARRY>
 __global__ void __launch_bounds__(TX * TY) k_fused_z(DevParams p, StepIO io, int kchunk) {
     __shared__ FusedLds<3 * MODEL, TX, TY> lds;
-    fused_z_tile<MODEL, CLOSURE, TX, TY, CARRY>(p, io, kchunk, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, lds);
+    fused_z_tile<MODEL, CLOSURE, TX, TY, CARRY>(p, io, kchunk, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, lds,
+                                                FusedStore<MODEL>{io});
 }
 
 // Boundary cell number t of the step after the interior tiles (t enumerates

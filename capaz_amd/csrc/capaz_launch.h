@@ -13,10 +13,13 @@ enum class Entry { S, P_Xi_ro, U, RoS, HE, E, Newton, Boundary };
 // (k_fused_step), 1 = z-marching tiles (k_fused_z) of 64 x tile_y columns
 // and kchunk planes per workgroup that carry the next plane's coefficients
 // in registers, 2 = the same tiles recomputing them per plane.
+// adjoint_kernel: 1 = the adjoint passes on the same z-marching LDS tiles,
+// 0 = one thread per cell reading neighbours from memory.
 struct Tuning {
     int fused_kernel = 1;
     int tile_y = 8;
     int kchunk = 16;
+    int adjoint_kernel = 1;
 };
 
 // Optional events recorded around the fused step's main kernel (the

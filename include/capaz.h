@@ -163,6 +163,8 @@ int capaz_step(capaz_ctx* ctx, long first_step, long nsteps);
  *                   2: as 1, recomputing the centre coefficients per plane
  *   "tile_y"        4 or 8 (default 8)
  *   "kchunk"        planes per workgroup along z (default 16)
+ *   "adjoint_kernel" 1: capaz_adjoint's passes on z-marching LDS tiles
+ *                   (default); 0: one thread per cell
  *   "timing"        1: record HIP events around each fused step's main
  *                   kernel (read with capaz_kernel_time); 0: off (default).
  *                   Setting it resets the totals.
@@ -189,6 +191,18 @@ int capaz_stream(capaz_ctx* ctx, void** stream);
  * plane's size; the plane is laid out as the reference's HostBuffer. */
 int capaz_load_face(capaz_ctx* ctx, int field, int axis, int side, double* dbuf);
 int capaz_save_face(capaz_ctx* ctx, int field, int axis, int side, const double* dbuf);
+
+/* ---- Checkpoint / restart ---------------------------------------------------
+ * Replace save / restore (main.cpp:1035-1145) with a format that keeps every
+ * field the next step reads (the primary state, including the boundary
+ * values of P_n and P_g, plus m and K; T for the energy model), so a restored
+ * context continues bit for bit:
+ *   "CAPAZST1", int32 version (1), model, energy, Nx, Ny, Nz, int64 step,
+ *   int32 nfields, nfields int32 field ids, then nfields x Nx*Ny*Nz doubles.
+ * Loading requires the same model and grid; the intermediate fields are
+ * recomputed by the next step (as after capaz_set_field). */
+int capaz_save_state(capaz_ctx* ctx, const char* path, long step);
+int capaz_load_state(capaz_ctx* ctx, const char* path, long* step);
 
 /* ---- History matching: objective and discrete adjoint ----------------------
  * No counterpart in the reference.  The objective of nsteps fused steps from

@@ -40,10 +40,13 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(a).max(), np.abs(b).max(), 1e-300))
 
 
-def test_adjoint_matches_host_build_and_checkpointing_is_exact():
-    keys, st, lnK, obs, rng = problem(20, 17, 12, 31, 9)
+@pytest.mark.parametrize("kernel", [1, 0])
+@pytest.mark.parametrize("dims", [(20, 17, 12), (70, 19, 21), (9, 11, 1), (1, 12, 9)])
+def test_adjoint_matches_host_build_and_checkpointing_is_exact(kernel, dims):
+    keys, st, lnK, obs, rng = problem(*dims, 31, 9)
     J_h, gK_h, gm_h = emul.gradient(keys, st, 9, obs)
     d = capaz.Domain(keys, st)
+    d.tune(adjoint_kernel=kernel)
     assert d.objective(9, obs) == J_h
     grads = []
     for every in (0, 1, 2, 4, 9):
