@@ -281,6 +281,18 @@ CAPAZ_HD void adj_newton_tail(const DevParams& p, const AdjIO& a, int i, int j, 
     a.dSn[l] = lam[2] + lA[1] * ro[1] - lA[2] * ro[2];
 }
 
+// Pass 1 at interior cell (i, j, k) from the flux divergence recorded by the
+// forward step (div_a at div + a*stride): the densities are recomputed with
+// the step's own expressions, so A is the step's A to the bit.
+template <int IT>
+CAPAZ_HD void adj_newton_recorded(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l,
+                                  const double* div_rec, int64_t stride) {
+    const double Pw = a.Pw[l];
+    const double ro[3] = {ro_w_of(p, Pw), ro_n_of(p, Pw), ro_g_of(p, Pw)};
+    const double div[3] = {div_rec[l], div_rec[stride + l], div_rec[2 * stride + l]};
+    adj_newton_tail<IT>(p, a, i, j, k, l, ro, Pw, a.Sw[l], a.Sn[l], div);
+}
+
 // Pass 1 at interior cell (i, j, k), reading its neighbours from memory.
 template <int IT>
 CAPAZ_HD void adj_newton_cell(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l) {

@@ -21,6 +21,14 @@ __global__ void __launch_bounds__(kAdjBlock) k_adj_newton(DevParams p, AdjIO a) 
     adj_newton_cell<IT>(p, a, i, j, k, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz);
 }
 
+template <int IT>
+__global__ void __launch_bounds__(kAdjBlock) k_adj_newton_rec(DevParams p, AdjIO a, const double* div,
+                                                              int64_t stride) {
+    const int i = (int)blockIdx.x * kAdjBlock + (int)threadIdx.x, j = (int)blockIdx.y, k = (int)blockIdx.z;
+    if (i >= p.Nx || !interior3(p, i, j, k)) return;
+    adj_newton_recorded<IT>(p, a, i, j, k, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz, div, stride);
+}
+
 __global__ void __launch_bounds__(kAdjBlock) k_adj_coef(DevParams p, AdjIO a) {
     const int i = (int)blockIdx.x * kAdjBlock + (int)threadIdx.x, j = (int)blockIdx.y, k = (int)blockIdx.z;
     if (i >= p.Nx) return;
@@ -207,6 +215,18 @@ hipError_t launch_adjoint_step_tiled(const DevParams& p, const AdjIO& a, hipStre
     const dim3 g2((unsigned)((p.Nx + kTX - 1) / kTX), (unsigned)((p.Ny + kTY - 1) / kTY),
                   (unsigned)((p.Nz + kc2 - 1) / kc2));
     hipLaunchKernelGGL(k_adj_coef_z, g2, dim3(kTX, kTY), 0, s, p, a, kc2);
+    return hipGetLastError();
+}
+
+hipError_t launch_adjoint_step_recorded(const DevParams& p, const AdjIO& a, const double* div, int64_t stride,
+                                        hipStream_t s) {
+    const dim3 grid((unsigned)((p.Nx + kAdjBlock - 1) / kAdjBlock), (unsigned)p.Ny, (unsigned)p.Nz);
+    // the run-time iteration count: its loop holds fewer live registers
+    // (144 VGPRs) than the fully unrolled fixed counts, without scratch
+    hipLaunchKernelGGL(k_adj_newton_rec<0>, grid, dim3(kAdjBlock), 0, s, p, a, div, stride);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_adj_coef, grid, dim3(kAdjBlock), 0, s, p, a);
     return hipGetLastError();
 }
 

@@ -5,10 +5,10 @@
 // checkpoint of the primary state every C steps (C = round(sqrt(nsteps)) by
 // default) and the observed saturations at the sampled steps, then sweeps the
 // segments backwards: each segment is recomputed from its checkpoint with
-// the fused kernel (the same bits as the first pass), and its steps are
-// reversed with the two adjoint passes.  Device memory: (nsteps/C + C) states
-// of five fields plus 18 fields of adjoint work, held by the context between
-// calls.
+// the fused kernel (the same bits as the first pass), recording each cell's
+// flux divergence, and its steps are reversed with the two adjoint passes.
+// Device memory: (nsteps/C + C + 1) states of five fields, 3C recorded
+// fields and 18 fields of adjoint work, held by the context between calls.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -88,8 +88,9 @@ int workspace(capaz_ctx* c, size_t doubles, double** out) {
     return CAPAZ_OK;
 }
 
-// One fused step from the five arrays at `from` to those at `to` (stride apart).
-int fused_between(capaz_ctx* c, const double* from, double* to, size_t stride) {
+// One fused step from the five arrays at `from` to those at `to` (stride
+// apart); with div != NULL it also records the flux divergence (3 fields).
+int fused_between(capaz_ctx* c, const double* from, double* to, size_t stride, double* div = nullptr) {
     capaz::StepIO io;
     io.Pw = from;
     io.Sw = from + stride;
@@ -104,7 +105,10 @@ int fused_between(capaz_ctx* c, const double* from, double* to, size_t stride) {
     io.Sn1 = to + 2 * stride;
     io.Pn1 = to + 3 * stride;
     io.Pg1 = to + 4 * stride;
-    ADJ_HIP(c, capaz::launch_fused(c->p, io, c->tuning, c->stream));
+    if (div)
+        ADJ_HIP(c, capaz::launch_fused_record(c->p, io, c->tuning.kchunk, div, (int64_t)stride, c->stream));
+    else
+        ADJ_HIP(c, capaz::launch_fused(c->p, io, c->tuning, c->stream));
     return CAPAZ_OK;
 }
 
@@ -117,23 +121,51 @@ int copy_state(capaz_ctx* c, double* dst, size_t stride) {
 
 // Forward steps from the context's state with the observation gathers;
 // every `every` steps (if > 0) the state is copied to ckpt + q*5*stride.
-// seen: [nsamples][2][ncells] on the device.
+// seen: [nsamples][2][ncells] on the device.  With rec_seg != NULL the steps
+// from rec_from on are run by the recording kernel into the segment buffers
+// (states at rec_seg + (t - rec_from)*5*stride for t > rec_from, the
+// divergence of step t at rec_div + (t - rec_from)*3*stride), the way the
+// backward sweep recomputes a segment, and the context then takes the last
+// two states as its current and previous ones.
 int forward(capaz_ctx* c, long nsteps, const Obs& ob, const int64_t* d_cells, double* seen, long every, double* ckpt,
-            size_t stride) {
+            size_t stride, long rec_from = -1, double* rec_seg = nullptr, double* rec_div = nullptr) {
     int rc;
     if (every > 0 && (rc = copy_state(c, ckpt, stride))) return rc;
-    for (long t = 0; t < nsteps; ++t) {
+    const long n_plain = rec_seg ? rec_from : nsteps;
+    auto gather = [&](long t1, const double* Sw, const double* Sn) -> int {
+        const int s = ob.sample_of[(size_t)t1];
+        if (s >= 0)
+            ADJ_HIP(c, capaz::launch_obs_gather(Sw, Sn, d_cells, ob.ncells, seen + (size_t)s * 2 * ob.ncells,
+                                                seen + (size_t)s * 2 * ob.ncells + ob.ncells, c->stream));
+        return CAPAZ_OK;
+    };
+    for (long t = 0; t < n_plain; ++t) {
         if ((rc = capaz_step(c, t + 1, 1))) return rc;
         const long t1 = t + 1;
         if (every > 0 && t1 % every == 0 && t1 < nsteps &&
             (rc = copy_state(c, ckpt + (size_t)(t1 / every) * kState * stride, stride)))
             return rc;
-        const int s = ob.sample_of[(size_t)t1];
-        if (s >= 0)
-            ADJ_HIP(c, capaz::launch_obs_gather(c->cur[capaz::F_S_w], c->cur[capaz::F_S_n], d_cells, ob.ncells,
-                                                seen + (size_t)s * 2 * ob.ncells,
-                                                seen + (size_t)s * 2 * ob.ncells + ob.ncells, c->stream));
+        if ((rc = gather(t1, c->cur[capaz::F_S_w], c->cur[capaz::F_S_n]))) return rc;
     }
+    if (!rec_seg) return CAPAZ_OK;
+    // the recorded tail starts from the checkpoint just taken (or the initial copy)
+    const double* x0 = ckpt + (size_t)(rec_from / (every > 0 ? every : 1)) * kState * stride;
+    auto slot = [&](long t) -> double* {
+        return t == rec_from ? const_cast<double*>(x0) : rec_seg + (size_t)(t - rec_from) * kState * stride;
+    };
+    for (long t = rec_from; t < nsteps; ++t) {
+        if ((rc = fused_between(c, slot(t), slot(t + 1), stride, rec_div + (size_t)(t - rec_from) * 3 * stride)))
+            return rc;
+        if ((rc = gather(t + 1, slot(t + 1) + stride, slot(t + 1) + 2 * stride))) return rc;
+    }
+    const size_t bytes = (size_t)c->p.n * sizeof(double);
+    for (int f = 0; f < kState; ++f) {
+        ADJ_HIP(c, hipMemcpyAsync(c->cur[kStateField[f]], slot(nsteps) + f * stride, bytes, hipMemcpyDeviceToDevice,
+                                  c->stream));
+        ADJ_HIP(c, hipMemcpyAsync(c->alt[kStateField[f]], slot(nsteps - 1) + f * stride, bytes,
+                                  hipMemcpyDeviceToDevice, c->stream));
+    }
+    c->stale = true;  // intermediates follow from the previous state, as after capaz_step
     return CAPAZ_OK;
 }
 
@@ -195,15 +227,19 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
     const long nseg = (nsteps + C - 1) / C;
     const size_t S = field_stride(c->p.n);
     const size_t n_state = (size_t)kState * S;
-    // layout: checkpoints | segment states | lambda x2 | work (6) | grads (2) | seen | observed | cells
+    // layout: checkpoints | segment states (C + 1) | recorded divergence (C x 3) | lambda x2 | work (6) |
+    // grads (2) | seen | observed | cells
+    const bool recorded = c->tuning.adjoint_kernel == 2;
     const size_t ns = (size_t)ob.nsamples * 2 * ob.ncells;
-    const size_t total =
-        (size_t)nseg * n_state + (size_t)C * n_state + 2 * n_state + 8 * S + 2 * ns + (size_t)ob.ncells;
+    const size_t n_div = recorded ? (size_t)C * 3 * S : 0;
+    const size_t total = (size_t)nseg * n_state + (size_t)(C + 1) * n_state + n_div + 2 * n_state + 8 * S + 2 * ns +
+                         (size_t)ob.ncells;
     double* w;
     if ((rc = workspace(c, total, &w))) return rc;
     double* ckpt = w;
     double* seg = ckpt + (size_t)nseg * n_state;
-    double* lam[2] = {seg + (size_t)C * n_state, seg + (size_t)C * n_state + n_state};
+    double* divs = seg + (size_t)(C + 1) * n_state;
+    double* lam[2] = {divs + n_div, divs + n_div + n_state};
     double* work = lam[1] + n_state;
     double* grad = work + 6 * S;
     double* seen = grad + 2 * S;
@@ -218,8 +254,12 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
             want[(size_t)s * 2 * ob.ncells + ob.ncells + q] = o->S_n[(size_t)s * ob.ncells + q];
         }
     if (ns) ADJ_HIP(c, hipMemcpyAsync(d_want, want.data(), ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    // forward sweep (objective_value waits for it, so `want` outlives its copy)
-    if ((rc = forward(c, nsteps, ob, d_cells, seen, C, ckpt, S))) return rc;
+    // forward sweep (objective_value waits for it, so `want` outlives its copy);
+    // recording, it runs the last segment the way the backward sweep would
+    const long last0 = (nseg - 1) * C;
+    if ((rc = recorded ? forward(c, nsteps, ob, d_cells, seen, C, ckpt, S, last0, seg, divs)
+                       : forward(c, nsteps, ob, d_cells, seen, C, ckpt, S)))
+        return rc;
     if ((rc = objective_value(c, o, ob, seen, J))) return rc;
     auto obs_adjoint = [&](double* l, long step) -> int {
         const int s = ob.sample_of[(size_t)step];
@@ -239,8 +279,12 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
     for (long q = nseg - 1; q >= 0; --q) {
         const long t0 = q * C, t1 = std::min(nsteps, t0 + C);
         auto slot = [&](long t) { return t == t0 ? ckpt + (size_t)q * n_state : seg + (size_t)(t - t0) * n_state; };
-        for (long t = t0; t + 1 < t1; ++t)
-            if ((rc = fused_between(c, slot(t), slot(t + 1), S))) return rc;
+        // the segment's states (and, recording, every step's divergence: the
+        // last step's new state lands in the spare slot C)
+        const bool have = recorded && q == nseg - 1;  // recorded by the forward sweep
+        for (long t = t0; !have && (t + 1 < t1 || (recorded && t < t1)); ++t)
+            if ((rc = fused_between(c, slot(t), slot(t + 1), S, recorded ? divs + (size_t)(t - t0) * 3 * S : nullptr)))
+                return rc;
         for (long t = t1 - 1; t >= t0; --t) {
             const double* x = slot(t);
             AdjIO a;
@@ -272,8 +316,11 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
             a.lPg = l0 + 4 * S;
             a.g_lnK = grad;
             a.g_m = grad + S;
-            e = c->tuning.adjoint_kernel ? launch_adjoint_step_tiled(c->p, a, c->stream)
-                                         : launch_adjoint_step(c->p, a, c->stream);
+            if (recorded)
+                e = launch_adjoint_step_recorded(c->p, a, divs + (size_t)(t - t0) * 3 * S, (int64_t)S, c->stream);
+            else
+                e = c->tuning.adjoint_kernel == 1 ? launch_adjoint_step_tiled(c->p, a, c->stream)
+                                                  : launch_adjoint_step(c->p, a, c->stream);
             if (e != hipSuccess)
                 return hip_fail(c, e, "launch_adjoint_step");
             if (t >= 1 && (rc = obs_adjoint(l0, t))) return rc;

@@ -11,6 +11,10 @@ namespace capaz {
 // reading neighbours from memory, or the z-marching LDS tiles.
 hipError_t launch_adjoint_step(const DevParams& p, const AdjIO& a, hipStream_t s);
 hipError_t launch_adjoint_step_tiled(const DevParams& p, const AdjIO& a, hipStream_t s);
+// The same with pass 1 reading the flux divergence recorded by the forward
+// step (div_a at div + a*stride) instead of recomputing the stencil.
+hipError_t launch_adjoint_step_recorded(const DevParams& p, const AdjIO& a, const double* div, int64_t stride,
+                                        hipStream_t s);
 hipError_t launch_obs_gather(const double* Sw, const double* Sn, const int64_t* cells, int64_t ncells, double* out_w,
                              double* out_n, hipStream_t s);
 hipError_t launch_obs_adjoint(double* lSw, double* lSn, const int64_t* cells, int64_t ncells, const double* seen_w,

@@ -377,9 +377,9 @@ int capaz_tune(capaz_ctx* c, const char* key, int value) {
         c->tuning.fused_kernel = value;
     else if (k == "tile_y" && (value == 4 || value == 8))
         c->tuning.tile_y = value;
-    else if (k == "kchunk" && value >= 1 && value <= 1 << 20)
+    else if (k == "kchunk" && value >= 0 && value <= 1 << 20)
         c->tuning.kchunk = value;
-    else if (k == "adjoint_kernel" && (value == 0 || value == 1))
+    else if (k == "adjoint_kernel" && value >= 0 && value <= 2)
         c->tuning.adjoint_kernel = value;
     else if (k == "timing" && (value == 0 || value == 1)) {
         int rc;

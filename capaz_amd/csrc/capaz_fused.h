@@ -289,6 +289,29 @@ struct FusedStore {
     }
 };
 
+// FusedStore that also records the cell's flux divergence div_a (field a at
+// div + a*stride), for the adjoint's backward Newton.
+template <int MODEL>
+struct FusedStoreRecord {
+    FusedStore<MODEL> step;
+    double* div;
+    int64_t stride;
+    __device__ __forceinline__ void operator()(const DevParams& p, int i, int j, int k, int64_t l,
+                                               const CoefT<MODEL>& c, const double* d) const {
+#pragma unroll
+        for (int a = 0; a < MODEL; ++a) div[a * stride + l] = d[a];
+        step(p, i, j, k, l, c, d);
+    }
+};
+
+template <int MODEL, int CLOSURE, int TX, int TY, bool CARRY>
+__global__ void __launch_bounds__(TX * TY) k_fused_z_record(DevParams p, StepIO io, int kchunk, double* div,
+                                                           int64_t stride) {
+    __shared__ FusedLds<3 * MODEL, TX, TY> lds;
+    fused_z_tile<MODEL, CLOSURE, TX, TY, CARRY>(p, io, kchunk, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, lds,
+                                                FusedStoreRecord<MODEL>{FusedStore<MODEL>{io}, div, stride});
+}
+
 template <int MODEL, int CLOSURE, int TX, int TY, bool CARRY>
 __global__ void __launch_bounds__(TX * TY) k_fused_z(DevParams p, StepIO io, int kchunk) {
     __shared__ FusedLds<3 * MODEL, TX, TY> lds;

@@ -126,7 +126,7 @@ hipError_t launch_fused_step(const DevParams& p, const StepIO& io, hipStream_t s
 template <int MODEL, int CLOSURE, int TY, bool CARRY>
 hipError_t launch_z(const DevParams& p, const StepIO& io, int kchunk, hipStream_t s, const KernelEvents* ev) {
     const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
-    if (kchunk < 1) kchunk = 1;
+    kchunk = fused_kchunk(p, kchunk, TY);
     dim3 grid((nx + 63) / 64, (ny + TY - 1) / TY, (nz + kchunk - 1) / kchunk);
     hipError_t e;
     if (ev && (e = hipEventRecord(ev->begin, s)) != hipSuccess) return e;
@@ -164,6 +164,35 @@ hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, h
         return ev ? hipEventRecord(ev->end, s) : hipSuccess;
     }
     return p.model == 3 ? launch_z_model<3>(p, io, t, s, ev) : launch_z_model<2>(p, io, t, s, ev);
+}
+
+int fused_kchunk(const DevParams& p, int kchunk, int ty) {
+    if (kchunk >= 1) return kchunk;
+    // automatic: ~2048 workgroups (8 per CU), between 4 and 16 planes per tile
+    const int64_t nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
+    const int64_t txy = ((nx + 63) / 64) * ((ny + ty - 1) / ty);
+    int64_t kc = (nz * txy + 2047) / 2048;
+    return (int)(kc < 4 ? 4 : (kc > 16 ? 16 : kc));
+}
+
+hipError_t launch_fused_record(const DevParams& p, const StepIO& io, int kchunk, double* div, int64_t stride,
+                               hipStream_t s) {
+    if (p.model != 3 || p.closure == 0) return hipErrorInvalidValue;
+    const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
+    kchunk = fused_kchunk(p, kchunk, 8);
+    dim3 grid((nx + 63) / 64, (ny + 7) / 8, (nz + kchunk - 1) / kchunk);
+    if (p.corey_n == 2.) {
+        hipLaunchKernelGGL((k_fused_z_record<3, 2, 64, 8, true>), grid, dim3(64, 8), 0, s, p, io, kchunk, div, stride);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_fused_faces<3, 2>), grid_for(fused_face_cells(p)), dim3(256), 0, s, p, io);
+    } else {
+        hipLaunchKernelGGL((k_fused_z_record<3, 1, 64, 8, true>), grid, dim3(64, 8), 0, s, p, io, kchunk, div, stride);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_fused_faces<3, 1>), grid_for(fused_face_cells(p)), dim3(256), 0, s, p, io);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_dtm(const DevParams& p, const double* m, double* dtm, hipStream_t s) {

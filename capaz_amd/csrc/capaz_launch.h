@@ -11,15 +11,18 @@ enum class Entry { S, P_Xi_ro, U, RoS, HE, E, Newton, Boundary };
 
 // Launch shape of the fused step: fused_kernel 0 = one thread per cell
 // (k_fused_step), 1 = z-marching tiles (k_fused_z) of 64 x tile_y columns
-// and kchunk planes per workgroup that carry the next plane's coefficients
-// in registers, 2 = the same tiles recomputing them per plane.
-// adjoint_kernel: 1 = the adjoint passes on the same z-marching LDS tiles,
-// 0 = one thread per cell reading neighbours from memory.
+// and kchunk planes per workgroup (0: chosen from the grid) that carry the
+// next plane's coefficients in registers, 2 = the same tiles recomputing
+// them per plane.
+// adjoint_kernel: 2 = the backward Newton per cell from the flux divergence
+// recorded by the segment's forward recompute, then the coefficient pass per
+// cell; 1 = both passes on the z-marching LDS tiles; 0 = both per cell,
+// recomputing the neighbours' coefficients.
 struct Tuning {
     int fused_kernel = 1;
     int tile_y = 8;
-    int kchunk = 16;
-    int adjoint_kernel = 1;
+    int kchunk = 0;
+    int adjoint_kernel = 2;
 };
 
 // Optional events recorded around the fused step's main kernel (the
@@ -33,6 +36,13 @@ hipError_t launch_entry(Entry e, const DevParams& p, const DevFields& F, hipStre
 hipError_t launch_fused_step(const DevParams& p, const StepIO& io, hipStream_t s);
 hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s,
                         const KernelEvents* ev = nullptr);
+// Planes per tile of the fused z kernels: kchunk if >= 1, else chosen from
+// the grid so that small grids still fill the GPU.
+int fused_kchunk(const DevParams& p, int kchunk, int ty);
+// One fused three-phase step (Corey closures, 64 x 8 tiles) that also writes
+// each interior cell's flux divergence of phase a to div[a*stride + l].
+hipError_t launch_fused_record(const DevParams& p, const StepIO& io, int kchunk, double* div, int64_t stride,
+                               hipStream_t s);
 hipError_t launch_dtm(const DevParams& p, const double* m, double* dtm, hipStream_t s);
 hipError_t launch_copy_interior(const DevParams& p, const double* src, double* dst, hipStream_t s);
 hipError_t launch_m_zero(const DevParams& p, const double* m, int* flag, hipStream_t s);

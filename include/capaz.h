@@ -162,9 +162,12 @@ int capaz_step(capaz_ctx* ctx, long first_step, long nsteps);
  *                   with the plane's coefficients staged in LDS (default);
  *                   2: as 1, recomputing the centre coefficients per plane
  *   "tile_y"        4 or 8 (default 8)
- *   "kchunk"        planes per workgroup along z (default 16)
- *   "adjoint_kernel" 1: capaz_adjoint's passes on z-marching LDS tiles
- *                   (default); 0: one thread per cell
+ *   "kchunk"        planes per workgroup along z; 0 (default): chosen from
+ *                   the grid (16 on large grids, fewer on small ones)
+ *   "adjoint_kernel" capaz_adjoint's passes: 2 (default) the backward
+ *                   Newton from the flux divergence recorded while each
+ *                   segment is recomputed; 1 both passes on z-marching LDS
+ *                   tiles; 0 both one thread per cell
  *   "timing"        1: record HIP events around each fused step's main
  *                   kernel (read with capaz_kernel_time); 0: off (default).
  *                   Setting it resets the totals.

@@ -40,7 +40,7 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(a).max(), np.abs(b).max(), 1e-300))
 
 
-@pytest.mark.parametrize("kernel", [1, 0])
+@pytest.mark.parametrize("kernel", [2, 1, 0])
 @pytest.mark.parametrize("dims", [(20, 17, 12), (70, 19, 21), (9, 11, 1), (1, 12, 9)])
 def test_adjoint_matches_host_build_and_checkpointing_is_exact(kernel, dims):
     keys, st, lnK, obs, rng = problem(*dims, 31, 9)
@@ -58,10 +58,10 @@ def test_adjoint_matches_host_build_and_checkpointing_is_exact(kernel, dims):
         grads.append((gK, gm))
     for gK, gm in grads[1:]:
         assert bit_equal(gK, grads[0][0]) and bit_equal(gm, grads[0][1])
-    # the context holds the state after 9 steps
+    # the context holds the state after 9 steps, intermediates included
     ref = capaz.Domain(keys, st)
     ref.run(9)
-    for f in ("P_w", "S_w", "S_n"):
+    for f in ref.names:
         assert bit_equal(d.field(f), ref.field(f)), f
     d.close()
     ref.close()
