@@ -4,6 +4,8 @@
 // of each field) and are launched on the context's stream.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "capaz_bodies.h"
 #include "capaz_fused.h"
 #include "capaz_launch.h"
@@ -123,10 +125,45 @@ hipError_t launch_fused_step(const DevParams& p, const StepIO& io, hipStream_t s
     return hipGetLastError();
 }
 
+// Workgroups of a kernel resident at once on the device (cached per kernel;
+// one device type per process).
+template <auto Kernel>
+int resident_groups(int threads) {
+    static int cached = 0;
+    if (cached) return cached;
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, Kernel, threads, 0) != hipSuccess || per_cu < 1)
+        return 512;
+    cached = cus * per_cu;
+    return cached;
+}
+
+int fused_kchunk(const DevParams& p, int kchunk, int ty, int resident) {
+    if (kchunk >= 1) return kchunk;
+    // automatic: the planes per tile that minimise (rounds of resident
+    // workgroups) x (planes per tile + the tile's extra coefficient planes)
+    const int64_t nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
+    const int64_t txy = ((nx + 63) / 64) * ((ny + ty - 1) / ty);
+    int best = 16;
+    double best_cost = -1.;
+    for (int kc = 2; kc <= 16; ++kc) {
+        const int64_t groups = txy * ((nz + kc - 1) / kc);
+        const int64_t rounds = (groups + resident - 1) / resident;
+        const double cost = (double)rounds * (std::min<int64_t>(kc, nz) + 0.5);
+        if (best_cost < 0. || cost < best_cost) {
+            best_cost = cost;
+            best = kc;
+        }
+    }
+    return best;
+}
+
 template <int MODEL, int CLOSURE, int TY, bool CARRY>
 hipError_t launch_z(const DevParams& p, const StepIO& io, int kchunk, hipStream_t s, const KernelEvents* ev) {
     const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
-    kchunk = fused_kchunk(p, kchunk, TY);
+    kchunk = fused_kchunk(p, kchunk, TY, resident_groups<k_fused_z<MODEL, CLOSURE, 64, TY, CARRY>>(64 * TY));
     dim3 grid((nx + 63) / 64, (ny + TY - 1) / TY, (nz + kchunk - 1) / kchunk);
     hipError_t e;
     if (ev && (e = hipEventRecord(ev->begin, s)) != hipSuccess) return e;
@@ -166,20 +203,11 @@ hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, h
     return p.model == 3 ? launch_z_model<3>(p, io, t, s, ev) : launch_z_model<2>(p, io, t, s, ev);
 }
 
-int fused_kchunk(const DevParams& p, int kchunk, int ty) {
-    if (kchunk >= 1) return kchunk;
-    // automatic: ~2048 workgroups (8 per CU), between 4 and 16 planes per tile
-    const int64_t nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
-    const int64_t txy = ((nx + 63) / 64) * ((ny + ty - 1) / ty);
-    int64_t kc = (nz * txy + 2047) / 2048;
-    return (int)(kc < 4 ? 4 : (kc > 16 ? 16 : kc));
-}
-
 hipError_t launch_fused_record(const DevParams& p, const StepIO& io, int kchunk, double* div, int64_t stride,
                                hipStream_t s) {
     if (p.model != 3 || p.closure == 0) return hipErrorInvalidValue;
     const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
-    kchunk = fused_kchunk(p, kchunk, 8);
+    kchunk = fused_kchunk(p, kchunk, 8, resident_groups<k_fused_z_record<3, 2, 64, 8, true>>(512));
     dim3 grid((nx + 63) / 64, (ny + 7) / 8, (nz + kchunk - 1) / kchunk);
     if (p.corey_n == 2.) {
         hipLaunchKernelGGL((k_fused_z_record<3, 2, 64, 8, true>), grid, dim3(64, 8), 0, s, p, io, kchunk, div, stride);

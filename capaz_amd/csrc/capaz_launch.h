@@ -37,8 +37,9 @@ hipError_t launch_fused_step(const DevParams& p, const StepIO& io, hipStream_t s
 hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s,
                         const KernelEvents* ev = nullptr);
 // Planes per tile of the fused z kernels: kchunk if >= 1, else chosen from
-// the grid so that small grids still fill the GPU.
-int fused_kchunk(const DevParams& p, int kchunk, int ty);
+// the grid and the number of workgroups resident at once, so that small grids
+// fill the GPU without a nearly empty last round.
+int fused_kchunk(const DevParams& p, int kchunk, int ty, int resident);
 // One fused three-phase step (Corey closures, 64 x 8 tiles) that also writes
 // each interior cell's flux divergence of phase a to div[a*stride + l].
 hipError_t launch_fused_record(const DevParams& p, const StepIO& io, int kchunk, double* div, int64_t stride,
