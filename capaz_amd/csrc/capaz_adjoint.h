@@ -236,32 +236,35 @@ CAPAZ_HD void for_boundary_of(const DevParams& p, int i, int j, int k, Fn&& f) {
             }
 }
 
-// Pass 1 at interior cell (i, j, k), l its index, once the step's
-// densities ro, state (Pw, Sw, Sn) and flux divergence div of the cell are
-// known (IT: newton3_adjoint's).
+// The transpose of Border for one boundary cell b (row bj) whose inward cell
+// is l: folds b's adjoint lb = (P_w, S_w, S_n, P_n, P_g) into l's (P_w, S_w,
+// S_n) adjoint lam (border_p3 and Border_S, three-phase.cpp:345-427).
+CAPAZ_HD void border_fold(const DevParams& p, int bj, const double* lb, double* lam) {
+    double dw = 1., dn = 1., dg = 1.;  // d(P_w, P_n, P_g)_b / dP_w(l)
+    if (bj == 0) {
+        dw = p.r_bp_w_top_den;
+        dn = p.r_bp_n_top_den;
+        dg = p.r_bp_g_top_den;
+    } else if (bj == p.Ny - 1) {
+        dw = p.r_bp_w_bot_den;
+        dn = p.r_bp_n_bot_den;
+        dg = p.r_bp_g_bot_den;
+    }
+    lam[0] += dw * lb[0] + dn * lb[3] + dg * lb[4];
+    if (!(bj == 0 && p.source > 0)) {  // the injected saturations are constants
+        lam[1] += lb[1];
+        lam[2] += lb[2];
+    }
+}
+
+// Pass 1's backward Newton at interior cell l with lam = the adjoint of
+// x_{t+1}(l) including its boundary cells, once the step's densities ro,
+// state (Pw, Sw, Sn) and flux divergence div of the cell are known (IT:
+// newton3_adjoint's).  Writes mu, the direct adjoint d and the porosity
+// gradient of the cell.
 template <int IT>
-CAPAZ_HD void adj_newton_tail(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l, const double* ro,
-                              double Pw, double Sw, double Sn, const double* div) {
-    // lambda of x_{t+1}(l), plus the boundary cells copied from it
-    double lam[3] = {a.lPw1[l], a.lSw1[l], a.lSn1[l]};
-    for_boundary_of(p, i, j, k, [&](int bi, int bj, int bk) {
-        const int64_t lb = (int64_t)bi + (int64_t)bj * p.sy + (int64_t)bk * p.sz;
-        double dw = 1., dn = 1., dg = 1.;  // d(P_w, P_n, P_g)_b / dP_w(l), border_p3
-        if (bj == 0) {
-            dw = p.r_bp_w_top_den;
-            dn = p.r_bp_n_top_den;
-            dg = p.r_bp_g_top_den;
-        } else if (bj == p.Ny - 1) {
-            dw = p.r_bp_w_bot_den;
-            dn = p.r_bp_n_bot_den;
-            dg = p.r_bp_g_bot_den;
-        }
-        lam[0] += dw * a.lPw1[lb] + dn * a.lPn1[lb] + dg * a.lPg1[lb];
-        if (!(bj == 0 && p.source > 0)) {
-            lam[1] += a.lSw1[lb];
-            lam[2] += a.lSn1[lb];
-        }
-    });
+CAPAZ_HD void adj_newton_core(const DevParams& p, const AdjIO& a, int64_t l, const double* ro, double Pw, double Sw,
+                              double Sn, const double* div, double* lam) {
     const double Sg = 1. - Sw - Sn, dtm = a.dtm[l];
     double A[3];
     A[0] = ro[0] * Sw - dtm * div[0];
@@ -279,6 +282,20 @@ CAPAZ_HD void adj_newton_tail(const DevParams& p, const AdjIO& a, int i, int j, 
     a.dPw[l] = lam[0] + lA[0] * p.c_wb * Sw + lA[1] * p.c_nb * Sn + lA[2] * rg * Sg;
     a.dSw[l] = lam[1] + lA[0] * ro[0] - lA[2] * ro[2];
     a.dSn[l] = lam[2] + lA[1] * ro[1] - lA[2] * ro[2];
+}
+
+// Pass 1 at interior cell (i, j, k), l its index, reading lambda_{t+1} from
+// the arrays.
+template <int IT>
+CAPAZ_HD void adj_newton_tail(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l, const double* ro,
+                              double Pw, double Sw, double Sn, const double* div) {
+    double lam[3] = {a.lPw1[l], a.lSw1[l], a.lSn1[l]};
+    for_boundary_of(p, i, j, k, [&](int bi, int bj, int bk) {
+        const int64_t lb = (int64_t)bi + (int64_t)bj * p.sy + (int64_t)bk * p.sz;
+        const double v[5] = {a.lPw1[lb], a.lSw1[lb], a.lSn1[lb], a.lPn1[lb], a.lPg1[lb]};
+        border_fold(p, bj, v, lam);
+    });
+    adj_newton_core<IT>(p, a, l, ro, Pw, Sw, Sn, div, lam);
 }
 
 // Pass 1 at interior cell (i, j, k) from the flux divergence recorded by the
@@ -387,10 +404,10 @@ CAPAZ_HD void adj_coef_axis(const DevParams& p, const CoefD& c, const double* mu
 }
 
 // Pass 2, after the axes: the coefficients' adjoint chained to the state and
-// ln K of cell l, plus pass 1's share if the cell is interior.
-CAPAZ_HD void adj_coef_finish(const DevParams& p, const AdjIO& a, int64_t l, const CoefD& c, const double* lP,
-                              const double* lL, const double* lro) {
-    (void)p;
+// ln K of cell l, plus pass 1's share if the cell is interior: out =
+// lambda_t(l) (P_w, S_w, S_n, P_n, P_g); returns the cell's ln K gradient.
+CAPAZ_HD double adj_coef_lambda(const AdjIO& a, int64_t l, const CoefD& c, const double* lP, const double* lL,
+                                const double* lro, double* out) {
     double lPw = 0., lSw = 0., lSn = 0., lPn = 0., lPg = 0., glnK = 0.;
     for (int ph = 0; ph < 3; ++ph) {
         const double lXi = -lL[ph] * c.ro[ph];
@@ -411,11 +428,25 @@ CAPAZ_HD void adj_coef_finish(const DevParams& p, const AdjIO& a, int64_t l, con
         lSw += a.dSw[l];
         lSn += a.dSn[l];
     }
-    a.lPw[l] = lPw;
-    a.lSw[l] = lSw;
-    a.lSn[l] = lSn;
-    a.lPn[l] = lPn;
-    a.lPg[l] = lPg;
+    out[0] = lPw;
+    out[1] = lSw;
+    out[2] = lSn;
+    out[3] = lPn;
+    out[4] = lPg;
+    return glnK;
+}
+
+// ... stored into the lambda_t arrays and the ln K gradient.
+CAPAZ_HD void adj_coef_finish(const DevParams& p, const AdjIO& a, int64_t l, const CoefD& c, const double* lP,
+                              const double* lL, const double* lro) {
+    (void)p;
+    double v[5];
+    const double glnK = adj_coef_lambda(a, l, c, lP, lL, lro, v);
+    a.lPw[l] = v[0];
+    a.lSw[l] = v[1];
+    a.lSn[l] = v[2];
+    a.lPn[l] = v[3];
+    a.lPg[l] = v[4];
     a.g_lnK[l] += glnK;
 }
 

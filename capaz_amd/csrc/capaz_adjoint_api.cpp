@@ -232,8 +232,9 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
     const bool recorded = c->tuning.adjoint_kernel == 2;
     const size_t ns = (size_t)ob.nsamples * 2 * ob.ncells;
     const size_t n_div = recorded ? (size_t)C * 3 * S : 0;
-    const size_t total = (size_t)nseg * n_state + (size_t)(C + 1) * n_state + n_div + 2 * n_state + 8 * S + 2 * ns +
-                         (size_t)ob.ncells;
+    const size_t n_work = 6 * S;
+    const size_t total = (size_t)nseg * n_state + (size_t)(C + 1) * n_state + n_div + 2 * n_state + n_work + 2 * S +
+                         2 * ns + (size_t)ob.ncells;
     double* w;
     if ((rc = workspace(c, total, &w))) return rc;
     double* ckpt = w;
@@ -241,7 +242,7 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
     double* divs = seg + (size_t)(C + 1) * n_state;
     double* lam[2] = {divs + n_div, divs + n_div + n_state};
     double* work = lam[1] + n_state;
-    double* grad = work + 6 * S;
+    double* grad = work + n_work;
     double* seen = grad + 2 * S;
     double* d_want = seen + ns;
     int64_t* d_cells = reinterpret_cast<int64_t*>(d_want + ns);
@@ -275,6 +276,36 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
         (e = hipMemsetAsync(grad, 0, 2 * S * sizeof(double), c->stream)) != hipSuccess)
         return hip_fail(c, e, "hipMemsetAsync");
     if ((rc = obs_adjoint(lam[0], nsteps))) return rc;
+    auto io_at = [&](const double* x, double* w, const double* l1, double* l0) {
+        AdjIO a;
+        a.Pw = x;
+        a.Sw = x + S;
+        a.Sn = x + 2 * S;
+        a.Pn = x + 3 * S;
+        a.Pg = x + 4 * S;
+        a.m = c->cur[F_m];
+        a.K = c->cur[F_K];
+        a.dtm = c->d_dtm;
+        a.lPw1 = l1;
+        a.lSw1 = l1 + S;
+        a.lSn1 = l1 + 2 * S;
+        a.lPn1 = l1 + 3 * S;
+        a.lPg1 = l1 + 4 * S;
+        a.mu_w = w;
+        a.mu_n = w + S;
+        a.mu_g = w + 2 * S;
+        a.dPw = w + 3 * S;
+        a.dSw = w + 4 * S;
+        a.dSn = w + 5 * S;
+        a.lPw = l0;
+        a.lSw = l0 + S;
+        a.lSn = l0 + 2 * S;
+        a.lPn = l0 + 3 * S;
+        a.lPg = l0 + 4 * S;
+        a.g_lnK = grad;
+        a.g_m = grad + S;
+        return a;
+    };
     int cur = 0;
     for (long q = nseg - 1; q >= 0; --q) {
         const long t0 = q * C, t1 = std::min(nsteps, t0 + C);
@@ -286,36 +317,8 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
             if ((rc = fused_between(c, slot(t), slot(t + 1), S, recorded ? divs + (size_t)(t - t0) * 3 * S : nullptr)))
                 return rc;
         for (long t = t1 - 1; t >= t0; --t) {
-            const double* x = slot(t);
-            AdjIO a;
-            a.Pw = x;
-            a.Sw = x + S;
-            a.Sn = x + 2 * S;
-            a.Pn = x + 3 * S;
-            a.Pg = x + 4 * S;
-            a.m = c->cur[F_m];
-            a.K = c->cur[F_K];
-            a.dtm = c->d_dtm;
-            const double* l1 = lam[cur];
             double* l0 = lam[cur ^ 1];
-            a.lPw1 = l1;
-            a.lSw1 = l1 + S;
-            a.lSn1 = l1 + 2 * S;
-            a.lPn1 = l1 + 3 * S;
-            a.lPg1 = l1 + 4 * S;
-            a.mu_w = work;
-            a.mu_n = work + S;
-            a.mu_g = work + 2 * S;
-            a.dPw = work + 3 * S;
-            a.dSw = work + 4 * S;
-            a.dSn = work + 5 * S;
-            a.lPw = l0;
-            a.lSw = l0 + S;
-            a.lSn = l0 + 2 * S;
-            a.lPn = l0 + 3 * S;
-            a.lPg = l0 + 4 * S;
-            a.g_lnK = grad;
-            a.g_m = grad + S;
+            const AdjIO a = io_at(slot(t), work, lam[cur], l0);
             if (recorded)
                 e = launch_adjoint_step_recorded(c->p, a, divs + (size_t)(t - t0) * 3 * S, (int64_t)S, c->stream);
             else
