@@ -323,30 +323,36 @@ __global__ void __launch_bounds__(TX * TY) k_fused_z(DevParams p, StepIO io, int
 // the j faces (all i, k), then the x faces (j interior, all k), then the z
 // faces (i, j interior)); nothing for t past the last face cell.
 template <int MODEL, int CLOSURE>
-__device__ __forceinline__ void fused_face(const DevParams& p, const StepIO& io, int64_t t) {
+__device__ __forceinline__ void fused_face(const DevParams& p, const StepIO& io, int64_t t64) {
+    // 32-bit index arithmetic: the face cells of a grid of < 2^31 cells
     const bool hasx = p.Nx >= 3, hasz = p.Nz >= 3;
-    const int64_t nj = 2LL * p.Nx * p.Nz;
-    const int64_t nx = hasx ? 2LL * (p.Ny - 2) * p.Nz : 0;
-    const int ni = hasx ? p.Nx - 2 : p.Nx;
-    const int64_t nzf = hasz ? 2LL * ni * (p.Ny - 2) : 0;
+    const uint32_t Nx = (uint32_t)p.Nx, Ny2 = (uint32_t)(p.Ny - 2);
+    const uint32_t nj = 2u * Nx * (uint32_t)p.Nz;
+    const uint32_t nx = hasx ? 2u * Ny2 * (uint32_t)p.Nz : 0u;
+    const uint32_t ni = hasx ? Nx - 2u : Nx;
+    const uint32_t nzf = hasz ? 2u * ni * Ny2 : 0u;
+    if (t64 >= (int64_t)nj + nx + nzf) return;
+    uint32_t t = (uint32_t)t64;
     int i, j, k;
     if (t < nj) {
-        j = (t & 1) ? p.Ny - 1 : 0;
+        j = (t & 1u) ? p.Ny - 1 : 0;
         t >>= 1;
-        i = (int)(t % p.Nx);
-        k = (int)(t / p.Nx);
+        const uint32_t q = t / Nx;
+        i = (int)(t - q * Nx);
+        k = (int)q;
     } else if ((t -= nj) < nx) {
-        i = (t & 1) ? p.Nx - 1 : 0;
+        i = (t & 1u) ? p.Nx - 1 : 0;
         t >>= 1;
-        j = 1 + (int)(t % (p.Ny - 2));
-        k = (int)(t / (p.Ny - 2));
-    } else if ((t -= nx) < nzf) {
-        k = (t & 1) ? p.Nz - 1 : 0;
-        t >>= 1;
-        i = (hasx ? 1 : 0) + (int)(t % ni);
-        j = 1 + (int)(t / ni);
+        const uint32_t q = t / Ny2;
+        j = 1 + (int)(t - q * Ny2);
+        k = (int)q;
     } else {
-        return;
+        t -= nx;
+        k = (t & 1u) ? p.Nz - 1 : 0;
+        t >>= 1;
+        const uint32_t q = t / ni;
+        i = (hasx ? 1 : 0) + (int)(t - q * ni);
+        j = 1 + (int)q;
     }
     const int64_t l1 = inward(p, i, j, k);
     const double Pw = io.Pw1[l1], Sn = io.Sn1[l1];
