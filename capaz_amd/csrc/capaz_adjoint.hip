@@ -21,18 +21,34 @@ __global__ void __launch_bounds__(kAdjBlock) k_adj_newton(DevParams p, AdjIO a) 
     adj_newton_cell<IT>(p, a, i, j, k, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz);
 }
 
+// Cell number t of a box of ni x nj cells per plane starting at (i0, j0, k0),
+// x fastest (32-bit index arithmetic: grids of < 2^31 cells).
+__device__ __forceinline__ void box_cell(uint32_t t, uint32_t ni, uint32_t nj, int i0, int j0, int k0, int& i, int& j,
+                                         int& k) {
+    const uint32_t a = t / ni, b = a / nj;
+    i = i0 + (int)(t - a * ni);
+    j = j0 + (int)(a - b * nj);
+    k = k0 + (int)b;
+}
+
+// Interior cells, densely numbered (every lane of a wavefront has a cell).
 template <int IT>
-__global__ void __launch_bounds__(kAdjBlock) k_adj_newton_rec(DevParams p, AdjIO a, const double* div,
-                                                              int64_t stride) {
-    const int i = (int)blockIdx.x * kAdjBlock + (int)threadIdx.x, j = (int)blockIdx.y, k = (int)blockIdx.z;
-    if (i >= p.Nx || !interior3(p, i, j, k)) return;
+__global__ void __launch_bounds__(256) k_adj_newton_rec(DevParams p, AdjIO a, const double* div, int64_t stride,
+                                                        uint32_t ni, uint32_t nj, uint32_t total) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= total) return;
+    int i, j, k;
+    box_cell(t, ni, nj, p.Nx >= 3 ? 1 : 0, 1, p.Nz >= 3 ? 1 : 0, i, j, k);
     adj_newton_recorded<IT>(p, a, i, j, k, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz, div, stride);
 }
 
-__global__ void __launch_bounds__(kAdjBlock) k_adj_coef(DevParams p, AdjIO a) {
-    const int i = (int)blockIdx.x * kAdjBlock + (int)threadIdx.x, j = (int)blockIdx.y, k = (int)blockIdx.z;
-    if (i >= p.Nx) return;
-    adj_coef_cell(p, a, i, j, k, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz);
+// Every cell, in memory order.
+__global__ void __launch_bounds__(256) k_adj_coef(DevParams p, AdjIO a) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= (uint32_t)p.n) return;
+    int i, j, k;
+    box_cell(t, (uint32_t)p.Nx, (uint32_t)p.Ny, 0, 0, 0, i, j, k);
+    adj_coef_cell(p, a, i, j, k, (int64_t)t);
 }
 
 // out_w[c] = S_w[cells[c]], out_n[c] = S_n[cells[c]]
@@ -218,16 +234,22 @@ hipError_t launch_adjoint_step_tiled(const DevParams& p, const AdjIO& a, hipStre
     return hipGetLastError();
 }
 
+hipError_t launch_adjoint_coef(const DevParams& p, const AdjIO& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_coef, dim3((unsigned)((p.n + 255) / 256)), dim3(256), 0, s, p, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_adjoint_step_recorded(const DevParams& p, const AdjIO& a, const double* div, int64_t stride,
                                         hipStream_t s) {
-    const dim3 grid((unsigned)((p.Nx + kAdjBlock - 1) / kAdjBlock), (unsigned)p.Ny, (unsigned)p.Nz);
+    const uint32_t ni = p.Nx >= 3 ? p.Nx - 2 : 1, nj = p.Ny - 2, nk = p.Nz >= 3 ? p.Nz - 2 : 1;
+    const uint32_t total = ni * nj * nk;
     // the run-time iteration count: its loop holds fewer live registers
     // (144 VGPRs) than the fully unrolled fixed counts, without scratch
-    hipLaunchKernelGGL(k_adj_newton_rec<0>, grid, dim3(kAdjBlock), 0, s, p, a, div, stride);
+    hipLaunchKernelGGL(k_adj_newton_rec<0>, dim3((total + 255) / 256), dim3(256), 0, s, p, a, div, stride, ni, nj,
+                       total);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_adj_coef, grid, dim3(kAdjBlock), 0, s, p, a);
-    return hipGetLastError();
+    return launch_adjoint_coef(p, a, s);
 }
 
 hipError_t launch_adjoint_step(const DevParams& p, const AdjIO& a, hipStream_t s) {
@@ -242,8 +264,7 @@ hipError_t launch_adjoint_step(const DevParams& p, const AdjIO& a, hipStream_t s
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_adj_coef, grid, dim3(kAdjBlock), 0, s, p, a);
-    return hipGetLastError();
+    return launch_adjoint_coef(p, a, s);
 }
 
 hipError_t launch_obs_gather(const double* Sw, const double* Sn, const int64_t* cells, int64_t ncells, double* out_w,
