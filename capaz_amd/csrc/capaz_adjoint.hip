@@ -42,8 +42,10 @@ __global__ void __launch_bounds__(256) k_adj_newton_rec(DevParams p, AdjIO a, co
     adj_newton_recorded<IT>(p, a, i, j, k, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz, div, stride);
 }
 
-// Every cell, in memory order.
-__global__ void __launch_bounds__(256) k_adj_coef(DevParams p, AdjIO a) {
+// Every cell, in memory order.  Four waves per SIMD (128 VGPRs; the
+// compiler's own choice of 130 allows three) hide more of the neighbours'
+// load latency, at the cost of a 12-byte spill.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_adj_coef(DevParams p, AdjIO a) {
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= (uint32_t)p.n) return;
     int i, j, k;
