@@ -3,7 +3,7 @@
 filtration step at 256^3 on one MI355X, with the HBM roofline of the fused
 step kernel and the CPU oracle timed beside it.
 
-    python bench.py [--steps K] [--warmup W] [--n 256] [--no-cpu-baseline] [--no-ensemble] [--no-adjoint]
+    python bench.py [--steps K] [--warmup W] [--n 256] [--no-cpu-baseline] [--no-ensemble] [--no-adjoint] [--no-energy]
 
 A step is one fused time step (capaz_step: closures, upwind Darcy fluxes,
 mass update, 4-iteration 3x3 Newton per cell and the boundary faces) over all
@@ -209,6 +209,40 @@ def bench_adjoint(n: int = 96, nsteps: int = 400, directions: int = 20) -> dict:
                    **{f"median_rel_err_eps_{eps:g}": float(np.median(v)) for eps, v in errs.items()}}}
 
 
+def bench_energy(nx: int = 256, ny: int = 256, nz: int = 128, steps: int = 10) -> dict:
+    """The reference's three-phase model with the energy equation on the
+    256 x 256 x 128 grid (capaz_amd.workloads.energy_3ph): `steps` steps of
+    the entry-point sequence (the energy model has no fused kernel), with the
+    oracle timed beside it on all host threads, 2 steps of a 256 x 256 x 32
+    slab."""
+    from capaz_amd import capaz, workloads
+    from oracle import oracle
+
+    keys, st = workloads.energy_3ph(nx, ny, nz)
+    d = capaz.Domain(keys, st)
+    d.run(2)
+    d.sync()
+    t0 = time.perf_counter()
+    d.run(steps, first_step=3)
+    d.sync()
+    wall = time.perf_counter() - t0
+    d.close()
+    cells = nx * ny * nz
+    slab, cpu_steps = 32, 2
+    sk = dict(keys, NZ=slab)
+    o = oracle.OracleDomain(sk, {k: v[:slab].copy() for k, v in st.items()})
+    t1 = time.perf_counter()
+    o.run(cpu_steps)
+    cpu = time.perf_counter() - t1
+    return {"workload": f"three-phase + energy (reference deck and closures) {nx}x{ny}x{nz}, 3 Newton iterations",
+            "path": "entry-point sequence (8 kernels per step)",
+            "value": round(cells * steps / wall / 1e6, 1), "unit": "Mcell-steps/s",
+            "ms_per_step": round(wall * 1e3 / steps, 3),
+            "cpu": {"value": round(nx * ny * slab * cpu_steps / cpu / 1e6, 2), "unit": "Mcell-steps/s",
+                    "cores": oracle.threads(), "kind": "port",
+                    "sample": f"{cpu_steps} steps of a {nx}x{ny}x{slab} slab, {cpu:.2f} s"}}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,7 +252,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--only", choices=("ensemble", "adjoint"), default=None,
+    ap.add_argument("--no-energy", action="store_true")
+    ap.add_argument("--only", choices=("ensemble", "adjoint", "energy"), default=None,
                     help="run only this secondary measurement and print it")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a copy of known size, to scale "
@@ -239,7 +274,7 @@ def main() -> int:
         return 2
     if args.only:
         # one secondary measurement alone (for profiling), as its own JSON line
-        fn = {"ensemble": bench_ensemble, "adjoint": bench_adjoint}[args.only]
+        fn = {"ensemble": bench_ensemble, "adjoint": bench_adjoint, "energy": bench_energy}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
         return 0
 
@@ -314,6 +349,9 @@ def main() -> int:
     if not args.no_adjoint:
         print("bench: adjoint", file=sys.stderr, flush=True)
         out["adjoint"] = bench_adjoint()
+    if not args.no_energy:
+        print("bench: energy", file=sys.stderr, flush=True)
+        out["energy"] = bench_energy()
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(args.n)

@@ -359,10 +359,13 @@ __device__ __forceinline__ void fused_face(const DevParams& p, const StepIO& io,
     const double Sw = MODEL == 3 ? io.Sw1[l1] : 0.;
     double ro_eff = 0.;
     if constexpr (MODEL == 2) {
-        // ro_eff_gdy (main.cpp:255-267): this step's densities of the inward
-        // cell (from its previous state) with its new saturation
-        const CoefT<2> c1 = coef_at<2, CLOSURE>(p, io, l1, true);
-        ro_eff = (c1.ro[1] * Sn + c1.ro[0] * (1 - Sn)) * io.m[l1] * p.g * p.hy;
+        // ro_eff_gdy (main.cpp:255-267), read by the bottom face only: this
+        // step's densities of the inward cell (from its previous state) with
+        // its new saturation
+        if (j == p.Ny - 1) {
+            const CoefT<2> c1 = coef_at<2, CLOSURE>(p, io, l1, true);
+            ro_eff = (c1.ro[1] * Sn + c1.ro[0] * (1 - Sn)) * io.m[l1] * p.g * p.hy;
+        }
     }
     write_boundary<MODEL>(p, io, i, j, k, Pw, Sw, Sn, ro_eff);
 }

@@ -144,3 +144,32 @@ def history_matching(n: int = 96, nsteps: int = 400, seed: int = 1004, columns: 
     cells = (i_c[:, None] + j[None, :] * n + k_c[:, None] * n * n).ravel().astype(np.int64)
     steps = np.arange(every, nsteps + 1, every, dtype=np.int64)
     return keys, st, lnK_true, lnK_trial, {"cells": cells, "steps": steps}
+
+
+# The reference's Three-phase/defines.ini deck (values only).
+REF_3PH_DECK = dict(
+    MODEL=3, HX=5e-2, HY=5e-2, HZ=5e-2, DT=1e-3, TAU=2e-3, S_W_GR=0.4, S_N_GR=0.3, S_W_INIT=0.4,
+    S_N_INIT=0.3, SOURCE=0, L=1e-7, C_N=1000, C_W=1500, C_G=330, BETA_W=4.4e-7, BETA_N=1.0e-6,
+    BETA_G=9.2e-6, MU_W=1e-3, MU_N=1e-3, MU_G=1.84e-5, RO_W=1000, RO_N=850, RO_G=1.4, P_ATM=1.0e5,
+    G_CONST=9.807, ITERATIONS=7, LAMBDA_0=3.25, LAMBDA_1=3.25, P_D_NW_0=3125.0, P_D_NW_1=3125.0,
+    P_D_GN_0=1042.0, P_D_GN_1=1042.0, K_0=6.64e-11, K_1=7.15e-12, M_0=0.4, M_1=0.39, S_WR_0=0.0,
+    S_WR_1=0.0, S_NR_0=0.0, S_NR_1=0.0, S_GR_0=0.0, S_GR_1=0.0, Q=0.15e-1,
+)
+
+
+def energy_3ph(nx: int = 256, ny: int = 256, nz: int = 128, seed: int = 1002):
+    """The reference's three-phase model with the energy equation
+    (-DTHREE_PHASE -DENERGY; its Three-phase/defines.ini deck, reference
+    closures, 3 Newton iterations, dt = 1e-4 s) on an nx x ny x nz grid.
+    Initial state: hydrostatic water pressure, S = (0.4, 0.3, 0.3), T = 285 K
+    (three-phase.cpp:470-487), porosity 0.4(1 + 0.05 N(0,1)) clipped to
+    [0.3, 0.5], uniform K_0.  The reference has no two-phase energy model, so
+    this is its own energy model on the grid of the non-isothermal config."""
+    rng = np.random.default_rng(seed)
+    keys = dict(REF_3PH_DECK, ENERGY=1, ITERATIONS=3, DT=1e-4, NX=nx, NY=ny, NZ=nz)
+    shape = (nz, ny, nx)
+    j = np.arange(ny).reshape(1, ny, 1)
+    P = np.broadcast_to(keys["P_ATM"] + 1000.0 * keys["G_CONST"] * keys["HY"] * j, shape).copy()
+    fields = dict(P_w=P, P_n=P.copy(), P_g=P.copy(), S_w=np.full(shape, 0.4), S_n=np.full(shape, 0.3),
+                  T=np.full(shape, 285.0), m=_porosity_noise(rng, shape), K=np.full(shape, keys["K_0"]))
+    return keys, fields

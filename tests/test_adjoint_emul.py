@@ -49,9 +49,10 @@ def oracle_J(keys, st, lnK, m, nsteps, obs):
     return emul.objective(seen, obs)
 
 
-@pytest.mark.parametrize("dims,seed,nsteps", [((7, 6, 5), 11, 6), ((9, 5, 1), 12, 4), ((1, 8, 6), 13, 4)])
-def test_adjoint_gradient_matches_central_differences(dims, seed, nsteps):
-    keys, st, lnK, obs, rng = problem(*dims, seed, nsteps)
+@pytest.mark.parametrize("dims,seed,nsteps,corey_n", [((7, 6, 5), 11, 6, 2.0), ((9, 5, 1), 12, 4, 2.0),
+                                                      ((1, 8, 6), 13, 4, 2.0), ((7, 6, 5), 14, 5, 2.6)])
+def test_adjoint_gradient_matches_central_differences(dims, seed, nsteps, corey_n):
+    keys, st, lnK, obs, rng = problem(*dims, seed, nsteps, COREY_N=corey_n)
     J, g_lnK, g_m = emul.gradient(keys, st, nsteps, obs)
     assert J == oracle_J(keys, st, lnK, st["m"], nsteps, obs)  # same forward, same bits
     eps = 1e-4

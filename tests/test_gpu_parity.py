@@ -131,6 +131,21 @@ def test_headline_physics_bitwise_at_96():
         assert bit_equal(d.field(f), od.field(f)), (f, max_rel(d.field(f), od.field(f)))
 
 
+def test_energy_workload_vs_oracle():
+    """The energy benchmark's model (reference three-phase deck with the
+    energy equation) on a 20 x 18 x 14 grid: 1 step within 1e-12 and 6 steps
+    within 1e-9 of the oracle, every field."""
+    keys, st = workloads.energy_3ph(20, 18, 14)
+    od = oracle.OracleDomain(keys, st)
+    d = capaz.Domain(keys, st)
+    od.run(1)
+    d.run(1)
+    check_fields(d.fields(), od.fields(), TOL_STEP1, "energy step 1")
+    od.run(5, first_step=2)
+    d.run(5, first_step=2)
+    check_fields(d.fields(), od.fields(), TOL_RUN, "energy after 6 steps")
+
+
 def test_full_size_fused_equals_sequence():
     """256^3 headline workload: two fused steps and two entry-point sequences
     give identical bits in every field."""
