@@ -82,6 +82,22 @@ def test_batch_launch_shapes(shape, persistent):
             b.close()
 
 
+@pytest.mark.parametrize("model", ["2ph", "3ph"])
+def test_batch_reference_closures(model):
+    # the reference's own closures (Brooks-Corey / van Genuchten, with pow):
+    # still the single-domain bits, realization by realization
+    if model == "2ph":
+        keys = case("2ph", 8, 8, 8, SOURCE=20)
+    else:
+        keys = case("3ph", 8, 8, 8, SOURCE=1)
+    ms = members(keys, DIMS[:4], 700)
+    b = capaz.Batch(keys, ms)
+    b.run(5)
+    for r, m in enumerate(ms):
+        assert_same(b.domains[r].fields(), alone(keys, m, 5), f"realization {r} {m['dims']}")
+    b.close()
+
+
 def test_batch_corey_matches_oracle():
     keys = keys_of("3ph")
     ms = members(keys, [(12, 10, 9), (7, 11, 5)], 300)
