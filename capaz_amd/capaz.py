@@ -61,6 +61,8 @@ EXPORTS = {
     "capaz_objective": (ctypes.c_int, ["ctx", ctypes.c_long, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "capaz_adjoint": (ctypes.c_int, ["ctx", ctypes.c_long, ctypes.c_void_p, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_double), "dbl_p", "dbl_p"]),
+    "capaz_tangent": (ctypes.c_int, ["ctx", ctypes.c_long, ctypes.c_void_p, "dbl_p", "dbl_p",
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "capaz_batch_create": (ctypes.c_int, ["params_p", ctypes.c_int, ctypes.POINTER(ctypes.c_int), "ctx_pp"]),
     "capaz_batch_destroy": (ctypes.c_int, ["ctx"]),
     "capaz_batch_last_error": (ctypes.c_char_p, ["ctx"]),
@@ -289,6 +291,17 @@ class Domain:
                                          gK.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                          gm.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "capaz_adjoint")
         return J.value, gK, gm
+
+    def tangent(self, nsteps: int, obs: dict, dlnK, dm):
+        """capaz_tangent: (J, dJ along (dlnK, dm)) after nsteps fused steps from the current state."""
+        o, keep = make_observations(obs["cells"], obs["steps"], obs["S_w"], obs["S_n"])
+        arr = lambda v: None if v is None else np.ascontiguousarray(np.broadcast_to(v, self.shape), dtype=np.float64)
+        vK, vm = arr(dlnK), arr(dm)
+        ptr = lambda v: v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if v is not None else None
+        J, dJ = ctypes.c_double(), ctypes.c_double()
+        self._check(load().capaz_tangent(self._ctx, nsteps, ctypes.byref(o), ptr(vK), ptr(vm), ctypes.byref(J),
+                                         ctypes.byref(dJ)), "capaz_tangent")
+        return J.value, dJ.value
 
     def kernel_time(self) -> tuple[float, int]:
         """(total ms, launches) of the fused step's main kernel since timing was set."""

@@ -72,7 +72,8 @@ struct CoefD {
     bool inner;  // P_n = P_g = P_w (three-phase.cpp:225-235)
 };
 
-CAPAZ_HD CoefD coef_d(const DevParams& p, const AdjIO& a, int64_t l, bool inner) {
+template <class IO>
+CAPAZ_HD CoefD coef_d(const DevParams& p, const IO& a, int64_t l, bool inner) {
     CoefD c;
     const double Pw = a.Pw[l], Sw = a.Sw[l], Sn = a.Sn[l], K = a.K[l];
     c.inner = inner;
@@ -104,7 +105,8 @@ CAPAZ_HD CoefD coef_d(const DevParams& p, const AdjIO& a, int64_t l, bool inner)
 }
 
 // Values only (neighbours in the gathers).
-CAPAZ_HD Coef coef_v(const DevParams& p, const AdjIO& a, int64_t l, bool inner) {
+template <class IO>
+CAPAZ_HD Coef coef_v(const DevParams& p, const IO& a, int64_t l, bool inner) {
     Coef c;
     const double Pw = a.Pw[l], Sw = a.Sw[l], Sn = a.Sn[l], K = a.K[l];
     c.P[0] = Pw;
@@ -483,6 +485,155 @@ CAPAZ_HD void adj_coef_cell(const DevParams& p, const AdjIO& a, int i, int j, in
         adj_coef_axis(p, c, mu_self, adj_nb_at(p, a, i, j, k - 1), adj_nb_at(p, a, i, j, k + 1), p.r_hz, false, lP,
                       lL, lro);
     adj_coef_finish(p, a, l, c, lP, lL, lro);
+}
+
+// ---- tangent-linear step -------------------------------------------------------
+//
+// The forward-mode derivative of the same step: given x_t, its tangent dx_t
+// and a parameter direction (d ln K, dm), dx_{t+1}.  Every partial is the one
+// the adjoint transposes (CoefD, directed_d with the mean slope at a zero
+// driving force, the Newton matrix's columns J - (dM/dx_j) u), so
+// <dJ from the tangent> = <gradient, direction> holds to rounding.
+
+struct TanIO {
+    const double *Pw, *Sw, *Sn, *Pn, *Pg, *m, *K, *dtm;  // x_t and the parameters
+    const double *dPw, *dSw, *dSn, *dPn, *dPg;           // dx_t
+    const double *dlnK, *dm;                             // parameter direction
+    double *dPw1, *dSw1, *dSn1, *dPn1, *dPg1;            // dx_{t+1}
+};
+
+// A cell's coefficients and their tangents.
+CAPAZ_HD void coef_tan(const DevParams& p, const TanIO& a, int64_t l, bool inner, Coef& c, Coef& dc) {
+    const CoefD d = coef_d(p, a, l, inner);
+    const double dP[3] = {a.dPw[l], inner ? a.dPw[l] : a.dPn[l], inner ? a.dPw[l] : a.dPg[l]};
+    const double dSw = a.dSw[l], dSn = a.dSn[l], dlnK = a.dlnK[l];
+    for (int ph = 0; ph < 3; ++ph) {
+        c.P[ph] = d.P[ph];
+        c.Xi[ph] = d.Xi[ph];
+        c.ro[ph] = d.ro[ph];
+        dc.P[ph] = dP[ph];
+        dc.Xi[ph] = d.dXi_dSw[ph] * dSw + d.dXi_dSn[ph] * dSn + d.Xi[ph] * dlnK;
+        dc.ro[ph] = d.dro_dP[ph] * dP[ph];
+    }
+}
+
+// Tangent of one cell's flux of phase ph along an axis (c the cell, m / q its
+// minus / plus neighbours; grav: the y axis).
+CAPAZ_HD double flux_tan(const DevParams& p, int ph, const Coef& c, const Coef& dc, const Coef& m, const Coef& dm,
+                         const Coef& q, const Coef& dq, double rh, bool grav) {
+    const double gr = grav ? p.g * c.ro[ph] : 0., dgr = grav ? p.g * dc.ro[ph] : 0.;
+    const double x1 = -((c.P[ph] - m.P[ph]) * rh) + gr, x2 = -((q.P[ph] - c.P[ph]) * rh) + gr;
+    const double dx1 = -((dc.P[ph] - dm.P[ph]) * rh) + dgr, dx2 = -((dq.P[ph] - dc.P[ph]) * rh) + dgr;
+    const double Lc = -c.Xi[ph] * c.ro[ph], Lm = -m.Xi[ph] * m.ro[ph], Lq = -q.Xi[ph] * q.ro[ph];
+    const double dLc = -(dc.Xi[ph] * c.ro[ph] + c.Xi[ph] * dc.ro[ph]);
+    const double dLm = -(dm.Xi[ph] * m.ro[ph] + m.Xi[ph] * dm.ro[ph]);
+    const double dLq = -(dq.Xi[ph] * q.ro[ph] + q.Xi[ph] * dq.ro[ph]);
+    const DirD d = directed_d(x1, x2, Lc, Lm, Lq, rh);
+    return d.dx1 * dx1 + d.dx2 * dx2 + d.dLc * dLc + d.dLm * dLm + d.dLp * dLq;
+}
+
+// Tangent of the fixed-iteration Newton: x' = x - M^-1 F, with A and its
+// tangent dA; dx (in: of the initial guess, out: of the result).
+template <int IT>
+CAPAZ_HD void newton3_tangent(const DevParams& p, double P0, double Sw0, double Sn0, const double* A, const double* dA,
+                              double* dx) {
+    constexpr int NX = IT > 0 ? IT : kMaxAdjIter;
+    const int it = IT > 0 ? IT : (p.iterations < kMaxAdjIter ? p.iterations : kMaxAdjIter);
+    double P = P0, Sw = Sw0, Sn = Sn0;
+    const double rg = p.ro0_g * p.r_P_atm;
+#if defined(__HIPCC__)
+#pragma unroll
+#endif
+    for (int k = 0; k < NX; ++k) {
+        if (k >= it) break;
+        double M[9];
+        newton3_matrix(p, P, Sw, Sn, M);
+        const double F[3] = {M[1] * Sw - A[0], M[5] * Sn - A[1], -M[7] * (1. - Sw - Sn) - A[2]};
+        const double MT[9] = {M[0], M[3], M[6], M[1], M[4], M[7], M[2], M[5], M[8]};
+        double u[3];
+        solve3_transposed(MT, F, u);
+        // (J - (dM/dx_j) u) dx - dA, columns as in newton3_adjoint
+        const double u0 = u[0], u1 = u[1], u2 = u[2], gP = rg * P;
+        const double cP0 = M[0] - p.c_wb * u1, cP1 = M[3] - p.c_nb * u2, cP2 = M[6] + rg * (u1 + u2);
+        const double cW0 = M[1] - p.c_wb * u0, cW2 = -gP + rg * u0;
+        const double cN1 = M[5] - (p.c_nb * u0 + p.ro0_n * u1), cN2 = -gP + rg * u0;
+        const double r[3] = {cP0 * dx[0] + cW0 * dx[1] - dA[0], cP1 * dx[0] + cN1 * dx[2] - dA[1],
+                             cP2 * dx[0] + cW2 * dx[1] + cN2 * dx[2] - dA[2]};
+        double du[3];
+        solve3_transposed(MT, r, du);
+        dx[0] -= du[0];
+        dx[1] -= du[1];
+        dx[2] -= du[2];
+        P -= u0;
+        Sw -= u1;
+        Sn -= u2;
+    }
+}
+
+// Interior cell (i, j, k), l its index: dx_{t+1}(l).
+template <int IT>
+CAPAZ_HD void tan_cell(const DevParams& p, const TanIO& a, int i, int j, int k, int64_t l) {
+    Coef c, dc;
+    coef_tan(p, a, l, true, c, dc);
+    double div[3] = {0., 0., 0.}, ddiv[3] = {0., 0., 0.};
+    auto nb = [&](int ii, int jj, int kk, Coef& n, Coef& dn) {
+        const int64_t q = (int64_t)ii + (int64_t)jj * p.sy + (int64_t)kk * p.sz;
+        coef_tan(p, a, q, interior3(p, ii, jj, kk), n, dn);
+    };
+    Coef xm = c, xp = c, zm = c, zp = c, ym, yp, dxm{}, dxp{}, dzm{}, dzp{}, dym, dyp;
+    if (!(p.Nx < 2)) {
+        nb(i - 1, j, k, xm, dxm);
+        nb(i + 1, j, k, xp, dxp);
+    }
+    if (!(p.Nz < 2)) {
+        nb(i, j, k - 1, zm, dzm);
+        nb(i, j, k + 1, zp, dzp);
+    }
+    nb(i, j - 1, k, ym, dym);
+    nb(i, j + 1, k, yp, dyp);
+    for (int ph = 0; ph < 3; ++ph) {
+        div[ph] = coef_divergence(p, ph, c, xm, xp, ym, yp, zm, zp);
+        if (!(p.Nx < 2)) ddiv[ph] += flux_tan(p, ph, c, dc, xm, dxm, xp, dxp, p.r_hx, false);
+        ddiv[ph] += flux_tan(p, ph, c, dc, ym, dym, yp, dyp, p.r_hy, true);
+        if (!(p.Nz < 2)) ddiv[ph] += flux_tan(p, ph, c, dc, zm, dzm, zp, dzp, p.r_hz, false);
+    }
+    const double Sw = a.Sw[l], Sn = a.Sn[l], Sg = 1. - Sw - Sn, dtm = a.dtm[l];
+    const double dSw = a.dSw[l], dSn = a.dSn[l];
+    const double ddtm = -dtm / a.m[l] * a.dm[l];
+    const double A[3] = {c.ro[0] * Sw - dtm * div[0], c.ro[1] * Sn - dtm * div[1], c.ro[2] * Sg - dtm * div[2]};
+    const double dA[3] = {dc.ro[0] * Sw + c.ro[0] * dSw - ddtm * div[0] - dtm * ddiv[0],
+                          dc.ro[1] * Sn + c.ro[1] * dSn - ddtm * div[1] - dtm * ddiv[1],
+                          dc.ro[2] * Sg - c.ro[2] * (dSw + dSn) - ddtm * div[2] - dtm * ddiv[2]};
+    double dx[3] = {a.dPw[l], dSw, dSn};
+    newton3_tangent<IT>(p, a.Pw[l], Sw, Sn, A, dA, dx);
+    a.dPw1[l] = dx[0];
+    a.dSw1[l] = dx[1];
+    a.dSn1[l] = dx[2];
+    a.dPn1[l] = 0.;  // interior P_n, P_g follow P_w and are never read
+    a.dPg1[l] = 0.;
+}
+
+// Boundary cell (i, j, k) from its inward cell's new tangent (Border_S /
+// border_p3, three-phase.cpp:345-427).
+CAPAZ_HD void tan_boundary(const DevParams& p, const TanIO& a, int i, int j, int k) {
+    const int64_t l = (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz, l1 = inward(p, i, j, k);
+    double dw = 1., dn = 1., dg = 1.;
+    if (j == 0) {
+        dw = p.r_bp_w_top_den;
+        dn = p.r_bp_n_top_den;
+        dg = p.r_bp_g_top_den;
+    } else if (j == p.Ny - 1) {
+        dw = p.r_bp_w_bot_den;
+        dn = p.r_bp_n_bot_den;
+        dg = p.r_bp_g_bot_den;
+    }
+    const double d1 = a.dPw1[l1];
+    a.dPw1[l] = dw * d1;
+    a.dPn1[l] = dn * d1;
+    a.dPg1[l] = dg * d1;
+    const bool injected = j == 0 && p.source > 0;
+    a.dSw1[l] = injected ? 0. : a.dSw1[l1];
+    a.dSn1[l] = injected ? 0. : a.dSn1[l1];
 }
 
 }  // namespace capaz

@@ -53,6 +53,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     adj_coef_cell(p, a, i, j, k, (int64_t)t);
 }
 
+// Tangent-linear step: interior cells (dense numbering), then the faces.
+__global__ void __launch_bounds__(256) k_tan_interior(DevParams p, TanIO a, uint32_t ni, uint32_t nj, uint32_t total) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= total) return;
+    int i, j, k;
+    box_cell(t, ni, nj, p.Nx >= 3 ? 1 : 0, 1, p.Nz >= 3 ? 1 : 0, i, j, k);
+    tan_cell<0>(p, a, i, j, k, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz);
+}
+
+__global__ void __launch_bounds__(256) k_tan_faces(DevParams p, TanIO a) {
+    int i, j, k;
+    if (!face_cell(p, (int64_t)blockIdx.x * 256 + threadIdx.x, i, j, k)) return;
+    tan_boundary(p, a, i, j, k);
+}
+
 // out_w[c] = S_w[cells[c]], out_n[c] = S_n[cells[c]]
 __global__ void k_obs_gather(const double* Sw, const double* Sn, const int64_t* cells, int64_t ncells, double* out_w,
                              double* out_n) {
@@ -267,6 +282,17 @@ hipError_t launch_adjoint_step(const DevParams& p, const AdjIO& a, hipStream_t s
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_adjoint_coef(p, a, s);
+}
+
+hipError_t launch_tangent_step(const DevParams& p, const TanIO& a, hipStream_t s) {
+    const uint32_t ni = p.Nx >= 3 ? p.Nx - 2 : 1, nj = p.Ny - 2, nk = p.Nz >= 3 ? p.Nz - 2 : 1;
+    const uint32_t total = ni * nj * nk;
+    hipLaunchKernelGGL(k_tan_interior, dim3((total + 255) / 256), dim3(256), 0, s, p, a, ni, nj, total);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t nf = fused_face_cells(p);
+    hipLaunchKernelGGL(k_tan_faces, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, s, p, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_obs_gather(const double* Sw, const double* Sn, const int64_t* cells, int64_t ncells, double* out_w,

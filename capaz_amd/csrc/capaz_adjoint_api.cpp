@@ -211,6 +211,87 @@ int capaz_objective(capaz_ctx* c, long nsteps, const capaz_observations* o, doub
     return objective_value(c, o, ob, w, J);
 }
 
+int capaz_tangent(capaz_ctx* c, long nsteps, const capaz_observations* o, const double* dlnK, const double* dm,
+                  double* J, double* dJ) {
+    using namespace capaz;
+    int rc;
+    if (!c || !J || !dJ) return CAPAZ_ERR_INVALID;
+    if ((rc = detail::check_ctx(c))) return rc;
+    if ((rc = check_model(c))) return rc;
+    Obs ob;
+    if ((rc = check_obs(c, nsteps, o, &ob))) return rc;
+    if ((rc = detail::scan_m(c))) return rc;
+    if (c->m_has_zero) return fail(c, CAPAZ_ERR_STATE, "adjoint: cells with m == 0");
+    const size_t S = field_stride(c->p.n), n = (size_t)c->p.n;
+    const size_t ns = (size_t)ob.nsamples * 2 * ob.ncells;
+    // layout: dx x2 (5 fields each) | dlnK, dm | seen | dseen | cells
+    double* w;
+    if ((rc = workspace(c, 12 * S + 2 * ns + (size_t)ob.ncells, &w))) return rc;
+    double* dx[2] = {w, w + 5 * S};
+    double* dth = w + 10 * S;
+    double* seen = w + 12 * S;
+    double* dseen = seen + ns;
+    int64_t* d_cells = reinterpret_cast<int64_t*>(dseen + ns);
+    ADJ_HIP(c, hipMemsetAsync(w, 0, 12 * S * sizeof(double), c->stream));
+    if (dlnK) ADJ_HIP(c, hipMemcpyAsync(dth, dlnK, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (dm) ADJ_HIP(c, hipMemcpyAsync(dth + S, dm, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (ob.ncells)
+        ADJ_HIP(c, hipMemcpyAsync(d_cells, o->cells, ob.ncells * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
+    int cur = 0;
+    for (long t = 0; t < nsteps; ++t) {
+        // the tangent of step t reads x_t: queued before the step itself
+        TanIO a;
+        a.Pw = c->cur[F_P_w];
+        a.Sw = c->cur[F_S_w];
+        a.Sn = c->cur[F_S_n];
+        a.Pn = c->cur[F_P_n];
+        a.Pg = c->cur[F_P_g];
+        a.m = c->cur[F_m];
+        a.K = c->cur[F_K];
+        a.dtm = c->d_dtm;
+        const double* d0 = dx[cur];
+        double* d1 = dx[cur ^ 1];
+        a.dPw = d0;
+        a.dSw = d0 + S;
+        a.dSn = d0 + 2 * S;
+        a.dPn = d0 + 3 * S;
+        a.dPg = d0 + 4 * S;
+        a.dlnK = dth;
+        a.dm = dth + S;
+        a.dPw1 = d1;
+        a.dSw1 = d1 + S;
+        a.dSn1 = d1 + 2 * S;
+        a.dPn1 = d1 + 3 * S;
+        a.dPg1 = d1 + 4 * S;
+        ADJ_HIP(c, launch_tangent_step(c->p, a, c->stream));
+        if ((rc = capaz_step(c, t + 1, 1))) return rc;
+        cur ^= 1;
+        const int smp = ob.sample_of[(size_t)(t + 1)];
+        if (smp >= 0) {
+            const size_t off = (size_t)smp * 2 * ob.ncells;
+            ADJ_HIP(c, launch_obs_gather(c->cur[F_S_w], c->cur[F_S_n], d_cells, ob.ncells, seen + off,
+                                         seen + off + ob.ncells, c->stream));
+            ADJ_HIP(c, launch_obs_gather(dx[cur] + S, dx[cur] + 2 * S, d_cells, ob.ncells, dseen + off,
+                                         dseen + off + ob.ncells, c->stream));
+        }
+    }
+    if ((rc = objective_value(c, o, ob, seen, J))) return rc;
+    std::vector<double> hs(ns), hd(ns);
+    if (ns) {
+        ADJ_HIP(c, hipMemcpy(hs.data(), seen, ns * sizeof(double), hipMemcpyDeviceToHost));
+        ADJ_HIP(c, hipMemcpy(hd.data(), dseen, ns * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    double sum = 0.;
+    for (int s = 0; s < ob.nsamples; ++s)
+        for (int64_t q = 0; q < ob.ncells; ++q) {
+            const size_t b = (size_t)s * 2 * ob.ncells;
+            sum += (hs[b + q] - o->S_w[(size_t)s * ob.ncells + q]) * hd[b + q] +
+                   (hs[b + ob.ncells + q] - o->S_n[(size_t)s * ob.ncells + q]) * hd[b + ob.ncells + q];
+        }
+    *dJ = sum;
+    return CAPAZ_OK;
+}
+
 int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int checkpoint_every, double* J,
                   double* grad_lnK, double* grad_m) {
     using namespace capaz;

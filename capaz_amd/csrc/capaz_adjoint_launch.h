@@ -15,6 +15,8 @@ hipError_t launch_adjoint_step_tiled(const DevParams& p, const AdjIO& a, hipStre
 // step (div_a at div + a*stride) instead of recomputing the stencil.
 hipError_t launch_adjoint_step_recorded(const DevParams& p, const AdjIO& a, const double* div, int64_t stride,
                                         hipStream_t s);
+// One tangent-linear step (interior cells, then the faces).
+hipError_t launch_tangent_step(const DevParams& p, const TanIO& a, hipStream_t s);
 hipError_t launch_obs_gather(const double* Sw, const double* Sn, const int64_t* cells, int64_t ncells, double* out_w,
                              double* out_n, hipStream_t s);
 hipError_t launch_obs_adjoint(double* lSw, double* lSn, const int64_t* cells, int64_t ncells, const double* seen_w,

@@ -319,21 +319,19 @@ __global__ void __launch_bounds__(TX * TY) k_fused_z(DevParams p, StepIO io, int
                                                 FusedStore<MODEL>{io});
 }
 
-// Boundary cell number t of the step after the interior tiles (t enumerates
-// the j faces (all i, k), then the x faces (j interior, all k), then the z
-// faces (i, j interior)); nothing for t past the last face cell.
-template <int MODEL, int CLOSURE>
-__device__ __forceinline__ void fused_face(const DevParams& p, const StepIO& io, int64_t t64) {
-    // 32-bit index arithmetic: the face cells of a grid of < 2^31 cells
+// Coordinates of boundary cell number t (t enumerates the j faces (all i,
+// k), then the x faces (j interior, all k), then the z faces (i, j
+// interior)); false for t past the last face cell.  32-bit index
+// arithmetic: the face cells of a grid of < 2^31 cells.
+__device__ __forceinline__ bool face_cell(const DevParams& p, int64_t t64, int& i, int& j, int& k) {
     const bool hasx = p.Nx >= 3, hasz = p.Nz >= 3;
     const uint32_t Nx = (uint32_t)p.Nx, Ny2 = (uint32_t)(p.Ny - 2);
     const uint32_t nj = 2u * Nx * (uint32_t)p.Nz;
     const uint32_t nx = hasx ? 2u * Ny2 * (uint32_t)p.Nz : 0u;
     const uint32_t ni = hasx ? Nx - 2u : Nx;
     const uint32_t nzf = hasz ? 2u * ni * Ny2 : 0u;
-    if (t64 >= (int64_t)nj + nx + nzf) return;
+    if (t64 >= (int64_t)nj + nx + nzf) return false;
     uint32_t t = (uint32_t)t64;
-    int i, j, k;
     if (t < nj) {
         j = (t & 1u) ? p.Ny - 1 : 0;
         t >>= 1;
@@ -354,6 +352,15 @@ __device__ __forceinline__ void fused_face(const DevParams& p, const StepIO& io,
         i = (hasx ? 1 : 0) + (int)(t - q * ni);
         j = 1 + (int)q;
     }
+    return true;
+}
+
+// Boundary cell number t of the step after the interior tiles (face_cell's
+// numbering); nothing for t past the last face cell.
+template <int MODEL, int CLOSURE>
+__device__ __forceinline__ void fused_face(const DevParams& p, const StepIO& io, int64_t t64) {
+    int i, j, k;
+    if (!face_cell(p, t64, i, j, k)) return;
     const int64_t l1 = inward(p, i, j, k);
     const double Pw = io.Pw1[l1], Sn = io.Sn1[l1];
     const double Sw = MODEL == 3 ? io.Sw1[l1] : 0.;

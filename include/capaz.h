@@ -235,6 +235,15 @@ int capaz_objective(capaz_ctx* ctx, long nsteps, const capaz_observations* obs, 
 int capaz_adjoint(capaz_ctx* ctx, long nsteps, const capaz_observations* obs, int checkpoint_every, double* J,
                   double* grad_lnK, double* grad_m);
 
+/* J and its directional derivative dJ along (dlnK, dm) (host arrays of
+ * Nx*Ny*Nz doubles, either may be NULL for zero) by the tangent-linear model
+ * of the fused step, run alongside nsteps forward steps from the current
+ * state (which the context then holds).  Its partials are those capaz_adjoint
+ * transposes, so dJ equals the gradient's inner product with the direction
+ * to rounding. */
+int capaz_tangent(capaz_ctx* ctx, long nsteps, const capaz_observations* obs, const double* dlnK, const double* dm,
+                  double* J, double* dJ);
+
 /* ---- Batched ensembles ------------------------------------------------------
  * No counterpart in the reference (one domain per process there): an
  * ensemble of realizations of one physics, each with its own grid and fields,

@@ -79,6 +79,30 @@ int capaz_emul_adjoint_step(const capaz_params* u, double** state, double** lam1
     return 0;
 }
 
+// One tangent-linear step (capaz_adjoint.h) on host arrays: state = {Pw, Sw,
+// Sn, Pn, Pg, m, K, dtm} at t, dstate = dx_t {Pw, Sw, Sn, Pn, Pg}, dtheta =
+// {dlnK, dm}, dnext = dx_{t+1} (written).
+int capaz_emul_tangent_step(const capaz_params* u, double** state, double** dstate, double** dtheta, double** dnext) {
+    if (!capaz::validate(*u).empty()) return -1;
+    capaz::DevParams p = capaz::derive(*u);
+    capaz::TanIO a;
+    a.Pw = state[0]; a.Sw = state[1]; a.Sn = state[2]; a.Pn = state[3]; a.Pg = state[4];
+    a.m = state[5]; a.K = state[6]; a.dtm = state[7];
+    a.dPw = dstate[0]; a.dSw = dstate[1]; a.dSn = dstate[2]; a.dPn = dstate[3]; a.dPg = dstate[4];
+    a.dlnK = dtheta[0]; a.dm = dtheta[1];
+    a.dPw1 = dnext[0]; a.dSw1 = dnext[1]; a.dSn1 = dnext[2]; a.dPn1 = dnext[3]; a.dPg1 = dnext[4];
+    for (int k = 0; k < p.Nz; ++k)
+        for (int j = 0; j < p.Ny; ++j)
+            for (int i = 0; i < p.Nx; ++i)
+                if (capaz::interior3(p, i, j, k))
+                    capaz::tan_cell<0>(p, a, i, j, k, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz);
+    for (int k = 0; k < p.Nz; ++k)
+        for (int j = 0; j < p.Ny; ++j)
+            for (int i = 0; i < p.Nx; ++i)
+                if (!capaz::interior3(p, i, j, k)) capaz::tan_boundary(p, a, i, j, k);
+    return 0;
+}
+
 int capaz_emul_reverse_matrix(double* a, int n) {
     if (n == -3) return capaz::reverse3(a);  // the no-pivot fast path with fallback
     if (n == -4) return capaz::reverse3<true>(a);  // the same for a[2] == +0

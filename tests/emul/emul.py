@@ -35,6 +35,7 @@ def lib():
         L.capaz_emul_fused_step.argtypes = [ctypes.POINTER(Params), PP, PP]
         L.capaz_emul_reverse_matrix.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.capaz_emul_adjoint_step.argtypes = [ctypes.POINTER(Params), PP, PP, PP, PP, PP]
+        L.capaz_emul_tangent_step.argtypes = [ctypes.POINTER(Params), PP, PP, PP, PP]
         L.capaz_emul_validate.argtypes = [ctypes.POINTER(Params)]
         L.capaz_emul_validate.restype = ctypes.c_char_p
         _lib = L
@@ -194,3 +195,36 @@ def gradient(keys: dict, fields: dict, nsteps: int, obs: dict):
         if t >= 1:
             add_obs(lam, t)
     return J, grad["lnK"], grad["m"]
+
+
+def tangent(keys: dict, fields: dict, nsteps: int, obs: dict, dlnK: np.ndarray, dm: np.ndarray):
+    """J and its directional derivative dJ along (dlnK, dm) by the host
+    build of the tangent-linear step, run alongside the forward step."""
+    p = params_from_keys(keys)
+    d = EmulDomain(keys, fields)
+    shape = d.shape
+    dx = {n: np.zeros(shape) for n in ADJ_STATE}
+    theta = [np.ascontiguousarray(dlnK, dtype=np.float64), np.ascontiguousarray(dm, dtype=np.float64)]
+    P = ctypes.POINTER(ctypes.c_double)
+    tab = lambda arrs: (P * len(arrs))(*[_ptr(a) for a in arrs])
+    seen, dseen = {}, {}
+    for t in range(nsteps):
+        st = [np.ascontiguousarray(d.arrays[n]) for n in ADJ_STATE] + [d.arrays["m"], d.arrays["K"],
+                                                                      p.dt / d.arrays["m"]]
+        nxt = {n: np.zeros(shape) for n in ADJ_STATE}
+        if lib().capaz_emul_tangent_step(ctypes.byref(p), tab(st), tab([dx[n] for n in ADJ_STATE]), tab(theta),
+                                         tab([nxt[n] for n in ADJ_STATE])):
+            raise RuntimeError("emul tangent step failed")
+        d.fused_step()
+        dx = nxt
+        if t + 1 in obs["steps"]:
+            seen[t + 1] = (d.arrays["S_w"].ravel().copy(), d.arrays["S_n"].ravel().copy())
+            dseen[t + 1] = (dx["S_w"].ravel().copy(), dx["S_n"].ravel().copy())
+    J = objective(seen, obs)
+    dJ = 0.0
+    for s_i, step in enumerate(obs["steps"]):
+        Sw, Sn = seen[step]
+        dSw, dSn = dseen[step]
+        c = obs["cells"]
+        dJ += float(np.sum((Sw[c] - obs["S_w"][s_i]) * dSw[c] + (Sn[c] - obs["S_n"][s_i]) * dSn[c]))
+    return J, dJ

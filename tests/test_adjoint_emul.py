@@ -114,3 +114,19 @@ def test_adjoint_at_zero_driving_forces_matches_central_differences():
               - emul.objective(run(lnK0 - eps * vK, st["m"] - eps * vm), obs)) / (2 * eps)
         ad = float(np.sum(gK * vK) + np.sum(gm * vm))
         assert abs(fd - ad) <= 1e-5 * abs(fd), (fd, ad)
+
+
+@pytest.mark.parametrize("dims,seed,nsteps", [((7, 6, 5), 11, 6), ((9, 5, 1), 12, 4), ((1, 8, 6), 13, 4)])
+def test_tangent_linear_and_adjoint_agree_to_rounding(dims, seed, nsteps):
+    # the dot-product test: the tangent-linear step's dJ along (v_K, v_m)
+    # equals the adjoint gradient's <g, v>; any error in the transpose (a
+    # missed term, a transposed index) shows as O(1), rounding as ~1e-15
+    keys, st, lnK, obs, rng = problem(*dims, seed, nsteps)
+    J, g_lnK, g_m = emul.gradient(keys, st, nsteps, obs)
+    for _ in range(3):
+        vK = rng.standard_normal(lnK.shape)
+        vm = 0.05 * rng.standard_normal(lnK.shape)
+        Jt, dJ = emul.tangent(keys, st, nsteps, obs, vK, vm)
+        assert Jt == J
+        ad = float(np.sum(g_lnK * vK) + np.sum(g_m * vm))
+        assert abs(dJ - ad) <= 1e-12 * abs(dJ), (dJ, ad)
