@@ -137,7 +137,9 @@ def bench_adjoint(n: int = 96, nsteps: int = 400, directions: int = 20) -> dict:
     sampled every 50 steps, observations from a seeded true ln K, trial ln K
     at its mean): wall time of the objective alone (capaz_objective) and of
     objective + gradient (capaz_adjoint, sqrt(N) checkpoints), and central
-    differences of capaz_objective along random directions in (ln K, m)."""
+    differences of capaz_objective along random directions in (ln K, m),
+    each also against the tangent-linear model's exact directional derivative
+    (capaz_tangent)."""
     from capaz_amd import capaz, workloads
 
     keys, st, lnK_true, lnK0, ob = workloads.history_matching(n, nsteps)
@@ -182,10 +184,17 @@ def bench_adjoint(n: int = 96, nsteps: int = 400, directions: int = 20) -> dict:
     every = int(round(nsteps ** 0.5))
     rng = np.random.default_rng(2024)
     errs = {1e-6: [], 1e-5: [], 1e-4: [], 1e-3: []}
+    tan_errs, t_tan = [], None
     for _ in range(directions):
         vK = rng.standard_normal(lnK0.shape)
         vm = 0.02 * rng.standard_normal(lnK0.shape)
         ad = float(np.sum(gK * vK) + np.sum(gm * vm))
+        reset(lnK0, m0)
+        t0 = time.perf_counter()
+        _, dJ = d.tangent(nsteps, obs, vK, vm)
+        dt = time.perf_counter() - t0
+        t_tan = dt if t_tan is None else min(t_tan, dt)
+        tan_errs.append(abs(dJ - ad) / max(abs(dJ), 1e-300))
         for eps in errs:
             reset(lnK0 + eps * vK, m0 + eps * vm)
             jp = d.objective(nsteps, obs)
@@ -204,6 +213,9 @@ def bench_adjoint(n: int = 96, nsteps: int = 400, directions: int = 20) -> dict:
             "adjoint_recompute_stencil_tiled_ms": round(t_adj_tiled * 1e3, 2),
             "adjoint_one_segment_over_forward": round(t_adj1 / t_fwd, 2),
             "checkpointing_changes_gradient": bool(not (np.array_equal(gK, gK1) and np.array_equal(gm, gm1))),
+            "tangent": {"ms": round(t_tan * 1e3, 2),
+                        "max_rel_err_adjoint_vs_tangent": float(max(tan_errs)),
+                        "median_rel_err_adjoint_vs_tangent": float(np.median(tan_errs))},
             "fd": {"directions": directions, "direction": "ln K ~ N(0,1), m ~ 0.02 N(0,1) per cell",
                    **{f"max_rel_err_eps_{eps:g}": float(max(v)) for eps, v in errs.items()},
                    **{f"median_rel_err_eps_{eps:g}": float(np.median(v)) for eps, v in errs.items()}}}

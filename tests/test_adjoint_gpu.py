@@ -8,7 +8,10 @@ capaz_adjoint).
   * the gradient does not depend on the checkpoint interval (each segment
     is recomputed with the same bits);
   * directional derivatives match central differences of capaz_objective;
-  * afterwards the context holds the state after nsteps, as capaz_step.
+  * afterwards the context holds the state after nsteps, as capaz_step;
+  * the tangent-linear model (capaz_tangent) gives the host build's J and dJ,
+    and dJ along a direction equals the adjoint gradient's inner product with
+    it to rounding: the two are transposes of the same partial derivatives.
 """
 import numpy as np
 import pytest
@@ -96,6 +99,31 @@ def test_adjoint_directional_derivatives_match_central_differences():
         ad = float(np.sum(gm * vm))
         assert abs(fd - ad) <= 1e-4 * abs(fd), (fd, ad)
     d.close()
+
+
+@pytest.mark.parametrize("dims", [(20, 17, 12), (9, 11, 1), (1, 12, 9)])
+def test_tangent_matches_host_build_and_the_adjoint(dims):
+    keys, st, lnK, obs, rng = problem(*dims, 61, 8)
+    d = capaz.Domain(keys, st)
+    J, gK, gm = d.adjoint(8, obs)
+    for _ in range(3):
+        vK = rng.standard_normal(lnK.shape)
+        vm = 0.05 * rng.standard_normal(lnK.shape)
+        for f, v in st.items():
+            d.set_field(f, v)
+        Jt, dJ = d.tangent(8, obs, vK, vm)
+        J_h, dJ_h = emul.tangent(keys, st, 8, obs, vK, vm)
+        assert Jt == J == J_h
+        assert abs(dJ - dJ_h) <= 1e-12 * abs(dJ_h), (dJ, dJ_h)
+        ad = float(np.sum(gK * vK) + np.sum(gm * vm))
+        assert abs(dJ - ad) <= 1e-12 * max(abs(dJ), float(np.sum(np.abs(gK * vK)) + np.sum(np.abs(gm * vm)))), \
+            (dJ, ad)
+    ref = capaz.Domain(keys, st)
+    ref.run(8)
+    for f in ref.names:
+        assert bit_equal(d.field(f), ref.field(f)), f
+    d.close()
+    ref.close()
 
 
 def test_history_matching_workload_small():
