@@ -1,0 +1,324 @@
+"""The reference driver's inputs and outputs around a capaz_amd domain: its
+defines.ini input deck, its initial state, its time loop and its Tecplot plot
+files, so that a reference input deck runs end to end on the backend and
+writes plots that compare line by line with the reference program's.
+
+``read_defines`` parses a deck as ``read_defines`` does (main.cpp:1280-1639):
+a line whose first character is '#' is skipped; the key is everything before
+the first '='; the value is everything after it up to the first space (or the
+end of the line) and is read by C's atof (or atoi for the integer keys), so
+"DT=1e-3  # comment" reads 1e-3 and "DT=abc" reads 0.  Unknown keys are
+ignored and a repeated key keeps its last value.  P_D_0/P_D_1 are read only
+for the two-phase model and the gas/three-phase keys only for the
+three-phase model, as the reference's #ifdefs do.  Lines without '=' are
+skipped (the reference reads past the end of its buffer on them).
+
+``initial_state`` is data_initialization (two-phase.cpp:113-172,
+three-phase.cpp:429-488) plus load_permeability (main.cpp:918-1032, built
+with LOAD_K_FROM_FILE: ../noise.dat, then noise.dat, else K_0 everywhere).
+
+``run_deck`` is main() (main.cpp:7-68): plots at t = 0, then time steps
+time_counter = 1 .. timeX/dt with "t=%.3f" every PRINT_SCREEN steps and plots
+every SAVE_PLOTS steps.  The reference's restore from save/save.dat is not
+read (its driver never writes it: the save call, main.cpp:63, is commented
+out).
+
+``save_data_plots`` writes plots/S=%012.4f.dat as print_plots_top /
+print_plots / print_plot_row do for one domain (main.cpp:474-780): the same
+header lines, rows in (i, j, k) order with k fastest, y printed as
+(Ny - 1 - j) * hy and u_y negated, and the same printf formats (Python's
+%-formatting of doubles is C's).
+"""
+from __future__ import annotations
+
+import math
+import os
+import re
+import time
+
+import numpy as np
+
+# main.cpp:1327-1626; TWO_PHASE / THREE_PHASE blocks per model.
+COMMON_KEYS = ("HX", "HY", "HZ", "TAU", "DT", "C_W", "C_N", "L", "BETA_W", "BETA_N", "RO_W", "RO_N", "MU_W",
+               "MU_N", "G_CONST", "P_ATM", "LAMBDA_0", "LAMBDA_1", "M_0", "M_1", "S_WR_0", "S_WR_1", "Q",
+               "BACKGROUND_Pw", "BACKGROUND_Sn", "INJECTION_WELL_Pw", "INJECTION_WELL_Sn", "OUTPUT_WELL_Pw",
+               "OUTPUT_WELL_Sn", "K_0", "K_1", "S_N_GR", "SOURCE", "ITERATIONS", "TIMEX", "SAVE_PLOTS",
+               "PRINT_SCREEN", "NX", "NY", "NZ")
+TWO_PHASE_KEYS = ("P_D_0", "P_D_1")
+THREE_PHASE_KEYS = ("C_G", "BETA_G", "RO_G", "MU_G", "P_D_NW_0", "P_D_NW_1", "P_D_GN_0", "P_D_GN_1", "S_W_GR",
+                    "S_W_INIT", "S_N_INIT", "S_NR_0", "S_NR_1", "S_GR_0", "S_GR_1")
+INT_KEYS = frozenset(("SOURCE", "ITERATIONS", "SAVE_PLOTS", "PRINT_SCREEN", "NX", "NY", "NZ"))
+
+_FLOAT = re.compile(r"[ \t\n\v\f\r]*([+-]?(?:0[xX](?:[0-9a-fA-F]+\.?[0-9a-fA-F]*|\.[0-9a-fA-F]+)(?:[pP][+-]?\d+)?"
+                    r"|(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?|inf(?:inity)?|nan(?:\([0-9A-Za-z_]*\))?))",
+                    re.IGNORECASE)
+_INT = re.compile(r"[ \t\n\v\f\r]*([+-]?\d+)")
+
+
+def atof(s: str) -> float:
+    """C's atof: the longest numeric prefix after leading white space, 0 if none."""
+    m = _FLOAT.match(s)
+    if not m:
+        return 0.0
+    t = m.group(1)
+    body = t.lstrip("+-")
+    if body[:2].lower() == "0x":
+        # a hex mantissa without an exponent is still a hex float to strtod
+        v = float.fromhex(body if "p" in body.lower() else body + "p0")
+        return -v if t.startswith("-") else v
+    if body[:3].lower() == "nan":
+        return math.copysign(math.nan, -1.0 if t.startswith("-") else 1.0)
+    return float(t)
+
+
+def atoi(s: str) -> int:
+    """C's atoi: leading white space, a sign and the digits that follow, 0 if none."""
+    m = _INT.match(s)
+    return int(m.group(1)) if m else 0
+
+
+def parse_defines(text: str, model: int = 3) -> dict:
+    """Keys of a defines.ini text for `model` (2 or 3), with the reference's
+    reading rules (main.cpp:1299-1626)."""
+    known = set(COMMON_KEYS) | set(TWO_PHASE_KEYS if model == 2 else THREE_PHASE_KEYS)
+    out: dict = {}
+    for raw in text.splitlines(keepends=True):
+        # fgets reads at most 249 characters of a line at a time
+        for start in range(0, max(len(raw), 1), 249):
+            line = raw[start:start + 249]
+            if not line or line[0] == "#" or "=" not in line:
+                continue
+            name, rest = line.split("=", 1)
+            value = rest.split(" ", 1)[0]
+            if name in known:
+                out[name] = atoi(value) if name in INT_KEYS else atof(value)
+    return out
+
+
+def find_defines(workdir: str = ".") -> str:
+    """The deck the reference opens: DEFINES_FILE "..//defines.ini", else
+    "defines.ini" (main.cpp:1287-1298), relative to workdir."""
+    for name in (os.path.join("..", "defines.ini"), "defines.ini"):
+        path = os.path.join(workdir, name)
+        if os.path.isfile(path):
+            return path
+    raise FileNotFoundError(f'Not open file "defines.ini" (looked in {workdir}/.. and {workdir})')
+
+
+def read_defines(path: str, model: int = 3, energy: int = 0) -> dict:
+    """A deck file as backend keys: its defines.ini values plus MODEL and
+    ENERGY, which the reference fixes at compile time (-DTWO_PHASE,
+    -DTHREE_PHASE, -DENERGY)."""
+    if model not in (2, 3):
+        raise ValueError("model must be 2 or 3")
+    if energy and model != 3:
+        raise ValueError("the energy equation exists for the three-phase model only")
+    with open(path, "rt", errors="surrogateescape") as f:
+        keys = parse_defines(f.read(), model)
+    keys["MODEL"] = model
+    if energy:
+        keys["ENERGY"] = 1
+    return keys
+
+
+def load_permeability(keys: dict, workdir: str = ".", log=print) -> np.ndarray:
+    """K of shape (Nz, Ny, Nx) as load_permeability reads it under
+    LOAD_K_FROM_FILE (main.cpp:918-1032): ../noise.dat, then noise.dat, else
+    K_0 in every cell."""
+    from .io import load_noise_dat
+
+    nx, ny, nz = int(keys["NX"]), int(keys["NY"]), int(keys["NZ"])
+    for name in (os.path.join("..", "noise.dat"), "noise.dat"):
+        path = os.path.join(workdir, name)
+        if os.path.isfile(path):
+            return load_noise_dat(path, nx, ny, nz)
+    log('Not open file "noise.dat", using value from def.K[0]')
+    return np.full((nz, ny, nx), float(keys.get("K_0", 0.0)))
+
+
+def initial_state(keys: dict, workdir: str = ".", log=print) -> dict:
+    """The reference's initial fields for the deck (data_initialization plus
+    load_permeability), as (Nz, Ny, Nx) arrays under their field names."""
+    nx, ny, nz = int(keys["NX"]), int(keys["NY"]), int(keys["NZ"])
+    shape = (nz, ny, nx)
+    g = lambda k: float(keys.get(k, 0.0))
+    P_atm = g("P_ATM")
+    st = {"m": np.full(shape, g("M_0")), "K": load_permeability(keys, workdir, log)}
+    if int(keys.get("MODEL", 3)) == 2:
+        # two-phase.cpp:113-172: S_n = S_n_gr in the source square of the top
+        # face, else 0; P_w hydrostatic down the columns by ro_eff_gdy of the
+        # cell above (main.cpp:255-267)
+        src = int(keys.get("SOURCE", 0))
+        S_n = np.zeros(shape)
+        i = np.arange(nx).reshape(1, nx)
+        k = np.arange(nz).reshape(nz, 1)
+        top = (i >= nx // 2 - src) & (i <= nx // 2 + src) & (k >= nz // 2 - src) & (k <= nz // 2 + src)
+        S_n[:, 0, :] = np.where(top, g("S_N_GR"), 0.0)
+        P = np.empty(shape)
+        ro_w = np.empty(shape)
+        ro_n = np.empty(shape)
+        m = st["m"]
+        for j in range(ny):
+            if j == 0:
+                P[:, 0, :] = P_atm
+            else:
+                a = j - 1
+                P[:, j, :] = P[:, a, :] + (ro_n[:, a, :] * S_n[:, a, :] + ro_w[:, a, :] * (1 - S_n[:, a, :])) * \
+                    m[:, a, :] * g("G_CONST") * g("HY")
+            ro_w[:, j, :] = g("RO_W") * (1. + g("BETA_W") * (P[:, j, :] - P_atm))
+            ro_n[:, j, :] = g("RO_N") * (1. + g("BETA_N") * (P[:, j, :] - P_atm))
+        st.update(P_w=P, S_n=S_n, ro_w=ro_w, ro_n=ro_n)
+    else:
+        # three-phase.cpp:437-488: uniform S_init, P = P_atm for all phases
+        S_w, S_n = g("S_W_INIT"), g("S_N_INIT")
+        P = np.full(shape, P_atm)
+        st.update(S_w=np.full(shape, S_w), S_n=np.full(shape, S_n), S_g=np.full(shape, 1. - S_w - S_n), P_w=P,
+                  P_n=P.copy(), P_g=P.copy(), ro_w=np.full(shape, g("RO_W") * (1. + g("BETA_W") * (P_atm - P_atm))),
+                  ro_n=np.full(shape, g("RO_N") * (1. + g("BETA_N") * (P_atm - P_atm))),
+                  ro_g=np.full(shape, g("RO_G") * P_atm / P_atm))
+        if int(keys.get("ENERGY", 0)):
+            st["T"] = np.full(shape, 285.0)
+    return st
+
+
+def plot_name(t: float) -> str:
+    return "S=%012.4f.dat" % t
+
+
+def _layout(keys: dict):
+    """(VARIABLES line, ZONE line, row format, column builder) of
+    print_plots_top / print_plot_row (main.cpp:542-780) for the model and
+    grid."""
+    nx, ny, nz = int(keys["NX"]), int(keys["NY"]), int(keys["NZ"])
+    three, energy = int(keys.get("MODEL", 3)) == 3, int(keys.get("ENERGY", 0)) != 0
+    e2, e5, e3 = "%.2e", "%.5e", "%.3e"
+    q = lambda names: "VARIABLES = " + ",".join(f'"{n}"' for n in names) + " \n"
+    if nz < 2:
+        zone = f'ZONE T = "BIG ZONE", K={ny}, F = POINT\n' if nx < 2 else \
+            f'ZONE T = "BIG ZONE", K={nx},J={ny}, F = POINT\n'
+    else:
+        zone = f'ZONE T = "BIG ZONE", K={nx},J={ny},I={nz}, F = POINT\n'
+    if three:
+        if nz < 2 and nx < 2:
+            names = ["Y", "S_w", "S_n", "S_g", "P_w"] + (["T"] if energy else []) + ["uw_y", "un_y", "ug_y",
+                                                                                       "porosity"]
+            cols = lambda f, X, Y, Z: [Y, f["S_w"], f["S_n"], f["S_g"], f["P_w"]] + ([f["T"]] if energy else []) + \
+                [-f["uy_w"], -f["uy_n"], -f["uy_g"], f["m"]]
+            fmts = [e2] + [e5] * (len(names) - 1)
+        elif nz < 2:
+            names = ["X", "Y", "S_w", "S_n", "S_g", "P_w"] + (["T"] if energy else []) + \
+                ["uw_x", "uw_y", "un_x", "un_y", "ug_x", "ug_y", "porosity"]
+            cols = lambda f, X, Y, Z: [X, Y, f["S_w"], f["S_n"], f["S_g"], f["P_w"]] + \
+                ([f["T"]] if energy else []) + [f["ux_w"], -f["uy_w"], f["ux_n"], -f["uy_n"], f["ux_g"], -f["uy_g"],
+                                                f["m"]]
+            fmts = [e2, e2] + [e5] * (len(names) - 2)
+        else:
+            names = ["X", "Y", "Z", "S_w", "S_n", "S_g", "P_w"] + (["T"] if energy else []) + \
+                ["uw_x", "uw_y", "uw_z", "un_x", "un_y", "un_z", "ug_x", "ug_y", "ug_z", "porosity"]
+            cols = lambda f, X, Y, Z: [X, Z, Y, f["S_w"], f["S_n"], f["S_g"], f["P_w"]] + \
+                ([f["T"]] if energy else []) + [f["ux_w"], f["uz_w"], -f["uy_w"], f["ux_n"], f["uz_n"], -f["uy_n"],
+                                                f["ux_g"], f["uz_g"], -f["uy_g"], f["m"]]
+            fmts = [e2] * 3 + [e5] * (len(names) - 3)
+        head = q(names)
+    else:
+        # the two-phase header names one column fewer than its rows print
+        # when Nx < 2 and Nz < 2 (main.cpp:566, 759)
+        if nz < 2:
+            head = q(["Y", "S_w", "P_w", "u_y", "porosity"]) if nx < 2 else \
+                'VARIABLES = "X","Y","S_w","P_w","u_x", "u_y", "porosity" \n'
+            cols = lambda f, X, Y, Z: [X, Y, f["S_n"], f["P_w"], f["ux_n"], -f["uy_n"], f["m"]]
+            fmts = [e2, e2] + [e3] * 5
+        else:
+            head = 'VARIABLES = "X","Y","Z","S_n","P_w","u_x", "u_y", "u_z", "porosity" \n'
+            cols = lambda f, X, Y, Z: [X, Z, Y, f["S_n"], f["P_w"], f["ux_n"], f["uz_n"], -f["uy_n"], f["m"]]
+            fmts = [e2] * 3 + [e3] * 6
+    return head, zone, " ".join(fmts) + "\n", cols
+
+
+def plot_text(fields: dict, keys: dict, t: float) -> str:
+    """The text of one plot file for the fields (each (Nz, Ny, Nx))."""
+    nx, ny, nz = int(keys["NX"]), int(keys["NY"]), int(keys["NZ"])
+    head, zone, fmt, cols = _layout(keys)
+    # rows in the order of print_plots' i, j, k loops (k fastest)
+    order = lambda a: np.asarray(a, dtype=np.float64).transpose(2, 1, 0).reshape(-1)
+    f = {n: order(v) for n, v in fields.items()}
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    X = i.reshape(-1) * float(keys["HX"])
+    Y = (ny - 1 - j.reshape(-1)) * float(keys["HY"])
+    Z = k.reshape(-1) * float(keys["HZ"])
+    table = np.stack(cols(f, X, Y, Z), axis=1)
+    # negating a zero velocity prints "-0.00000e+00", as (-1)*u does in C
+    out = ['TITLE =  "Filtration in time=%5.2f" \n' % t, head, zone]
+    out.extend(fmt % tuple(r) for r in table.tolist())
+    return "".join(out)
+
+
+PLOT_FIELDS = ("S_w", "S_n", "S_g", "P_w", "T", "m", "ux_w", "uy_w", "uz_w", "ux_n", "uy_n", "uz_n", "ux_g",
+               "uy_g", "uz_g")
+
+
+def save_data_plots(domain, keys: dict, t: float, workdir: str = ".") -> str:
+    """Write plots/S=<t>.dat for the domain's current fields
+    (save_data_plots, main.cpp:474-539); returns the path."""
+    names = set(domain.names)
+    fields = {n: domain.field(n) for n in PLOT_FIELDS if n in names}
+    d = os.path.join(workdir, "plots")
+    os.makedirs(d, exist_ok=True)
+    path = os.path.join(d, plot_name(t))
+    with open(path, "wt") as fp:
+        fp.write(plot_text(fields, keys, t))
+    return path
+
+
+def run_deck(workdir: str = ".", model: int = 3, energy: int = 0, backend=None, overrides: dict | None = None,
+             fused: bool = False, log=print) -> dict:
+    """Run the deck found from workdir as the reference's main() does and
+    write its plots under workdir/plots.
+
+    backend(keys, fields) makes the domain (default capaz_amd.capaz.Domain);
+    it must offer call(entry, t), field(name), names and, for fused=True,
+    run(nsteps, first_step).  overrides are backend keys added to the deck
+    (CLOSURE, COREY_N).  fused=True advances with the fused step and runs the
+    entry-point sequence only on the steps that end in a plot, so that the
+    velocities and S_g in the plots are the ones the sequence leaves, as in
+    the reference (the fused step does not write them).  Returns the keys, the
+    number of steps and the plot paths."""
+    keys = read_defines(find_defines(workdir), model, energy)
+    if overrides:
+        keys.update(overrides)
+    if backend is None:
+        from .capaz import Domain as backend
+    name = {2: "Two phase filtration", 3: "Three phase filtration"}[model]
+    log(f"{name} by CAPAZ on 1 node(s).")
+    st = initial_state(keys, workdir, log)
+    d = backend(keys, st)
+    dt, timeX = float(keys["DT"]), float(keys.get("TIMEX", 0.0))
+    save_plots, print_screen = int(keys.get("SAVE_PLOTS", 0)), int(keys.get("PRINT_SCREEN", 0))
+    if save_plots <= 0 or print_screen <= 0:
+        raise ValueError("SAVE_PLOTS and PRINT_SCREEN must be positive")
+    if fused and energy:
+        raise ValueError("the energy model has no fused step")
+    plots = [save_data_plots(d, keys, 0.0, workdir)]
+    # main()'s loop runs while time_counter <= timeX / dt
+    ratio = timeX / dt
+    total = int(math.floor(ratio)) if math.isfinite(ratio) and ratio >= 1 else 0
+    t0 = time.perf_counter()
+    tc = 1
+    while tc <= total:
+        if fused and tc % save_plots:
+            # fused steps up to the step before the next plot step
+            end = min(total, (tc // save_plots + 1) * save_plots - 1)
+            for s_ in range(tc, end + 1):
+                if s_ % print_screen == 0:
+                    log("t=%.3f" % (s_ * dt))
+            d.run(end - tc + 1, first_step=tc)
+            tc = end + 1
+            continue
+        if tc % print_screen == 0:
+            log("t=%.3f" % (tc * dt))
+        d.call("time_step_function", tc * dt)
+        if tc % save_plots == 0:
+            plots.append(save_data_plots(d, keys, tc * dt, workdir))
+        tc += 1
+    log("Task time in seconds:\t%.2f" % (time.perf_counter() - t0))
+    return {"keys": keys, "steps": tc - 1, "plots": plots, "domain": d}
