@@ -21,16 +21,6 @@ __global__ void __launch_bounds__(kAdjBlock) k_adj_newton(DevParams p, AdjIO a) 
     adj_newton_cell<IT>(p, a, i, j, k, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz);
 }
 
-// Cell number t of a box of ni x nj cells per plane starting at (i0, j0, k0),
-// x fastest (32-bit index arithmetic: grids of < 2^31 cells).
-__device__ __forceinline__ void box_cell(uint32_t t, uint32_t ni, uint32_t nj, int i0, int j0, int k0, int& i, int& j,
-                                         int& k) {
-    const uint32_t a = t / ni, b = a / nj;
-    i = i0 + (int)(t - a * ni);
-    j = j0 + (int)(a - b * nj);
-    k = k0 + (int)b;
-}
-
 // Interior cells, densely numbered (every lane of a wavefront has a cell).
 template <int IT>
 __global__ void __launch_bounds__(256) k_adj_newton_rec(DevParams p, AdjIO a, const double* div, int64_t stride,

@@ -10,6 +10,7 @@
 // interior P_n / P_g); they are marked stale and recomputed from the previous
 // state -- which the second arrays still hold -- before anything reads them,
 // so every field always reads as after the same number of reference steps.
+// The energy model's step (six passes) works in place like the sequence.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -207,6 +208,7 @@ int capaz_create(const capaz_params* u, capaz_ctx** out) {
     if ((e = hipMalloc(&c->d_flag, sizeof(int))) != hipSuccess) return bail(e, "hipMalloc");
     const size_t bytes = (size_t)c->p.n * sizeof(double);
     if ((e = hipMalloc(&c->d_dtm, bytes)) != hipSuccess) return bail(e, "hipMalloc");
+    if (u->energy && (e = hipMalloc(&c->d_lam, bytes)) != hipSuccess) return bail(e, "hipMalloc");
     for (int f = 0; f < capaz::F_COUNT; ++f) {
         if (!field_exists(*u, f)) continue;
         if ((e = hipMalloc(&c->cur[f], bytes)) != hipSuccess) return bail(e, "hipMalloc");
@@ -232,6 +234,7 @@ int capaz_destroy(capaz_ctx* c) {
     }
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->d_dtm) (void)hipFree(c->d_dtm);
+    if (c->d_lam) (void)hipFree(c->d_lam);
     if (c->adj_mem) (void)hipFree(c->adj_mem);
     for (auto& e : c->ev) {
         if (e.begin) (void)hipEventDestroy(e.begin);
@@ -337,14 +340,24 @@ int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
     if ((rc = check_ctx(c))) return rc;
     if (nsteps < 0) return fail(c, CAPAZ_ERR_INVALID, "nsteps must be >= 0");
     if ((rc = scan_m(c))) return rc;
+    using namespace capaz;
+    if (c->user.energy && !c->m_has_zero && c->tuning.energy_kernel == 1) {
+        // the energy model's six passes, in place like the sequence
+        if ((rc = materialize(c))) return rc;
+        for (long s = 0; s < nsteps; ++s) {
+            const KernelEvents* ev;
+            if ((rc = next_events(c, &ev))) return rc;
+            CAPAZ_HIP(c, launch_energy_step(c->p, fields_of(c), c->d_lam, c->stream, ev));
+        }
+        return CAPAZ_OK;
+    }
     if (c->user.energy || c->m_has_zero) {
-        // No fused kernel for the energy model; interior cells with m == 0 keep
-        // the reference's stale-roS rule, which only the sequence reproduces.
+        // Interior cells with m == 0 keep the reference's stale-roS rule,
+        // which only the sequence reproduces.
         for (long s = first_step; s < first_step + nsteps; ++s)
             if ((rc = capaz_step_sequence(c, (double)s * c->user.dt))) return rc;
         return CAPAZ_OK;
     }
-    using namespace capaz;
     for (long s = 0; s < nsteps; ++s) {
         StepIO io;
         io.Pw = c->cur[F_P_w];
@@ -381,6 +394,8 @@ int capaz_tune(capaz_ctx* c, const char* key, int value) {
         c->tuning.kchunk = value;
     else if (k == "adjoint_kernel" && value >= 0 && value <= 2)
         c->tuning.adjoint_kernel = value;
+    else if (k == "energy_kernel" && (value == 0 || value == 1))
+        c->tuning.energy_kernel = value;
     else if (k == "timing" && (value == 0 || value == 1)) {
         int rc;
         if ((rc = check_ctx(c)) || (rc = drain_events(c))) return rc;
@@ -399,6 +414,7 @@ int capaz_get_tuning(const capaz_ctx* c, const char* key, int* value) {
     else if (k == "tile_y") *value = c->tuning.tile_y;
     else if (k == "kchunk") *value = c->tuning.kchunk;
     else if (k == "adjoint_kernel") *value = c->tuning.adjoint_kernel;
+    else if (k == "energy_kernel") *value = c->tuning.energy_kernel;
     else if (k == "timing") *value = c->timing ? 1 : 0;
     else return CAPAZ_ERR_INVALID;
     return CAPAZ_OK;

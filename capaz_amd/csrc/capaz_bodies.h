@@ -183,11 +183,21 @@ CAPAZ_HD double lambda_eff_at(const DevFields& F, int64_t l) {
     return lambda_eff(F.f[F_m][l], F.f[F_S_w][l], F.f[F_S_n][l], F.f[F_S_g][l], F.f[F_T][l]);
 }
 
-CAPAZ_HD double conduction(const DevFields& F, int64_t l, int64_t s, double h2, double rh2) {
+// lambda_eff of a cell: computed from the fields, or read from an array that
+// holds the same values computed earlier in the step.
+struct LamOfFields {
+    const DevFields& F;
+    CAPAZ_HD double operator()(int64_t l) const { return lambda_eff_at(F, l); }
+};
+struct LamOfArray {
+    const double* lam;
+    CAPAZ_HD double operator()(int64_t l) const { return lam[l]; }
+};
+
+template <class Lam>
+CAPAZ_HD double conduction(const DevFields& F, const Lam& lam, int64_t l, int64_t s, double h2, double rh2) {
     const double* T = F.f[F_T];
-    return exact_div(lambda_eff_at(F, l + s) * T[l + s] - 2 * lambda_eff_at(F, l) * T[l] +
-                         lambda_eff_at(F, l - s) * T[l - s],
-                     h2, rh2);
+    return exact_div(lam(l + s) * T[l + s] - 2 * lam(l) * T[l] + lam(l - s) * T[l - s], h2, rh2);
 }
 
 CAPAZ_HD double advection(const DevFields& F, const double* uw, const double* un, const double* ug, int64_t a,
@@ -198,14 +208,15 @@ CAPAZ_HD double advection(const DevFields& F, const double* uw, const double* un
            rg[a] * Hg[a] * ug[a] - rg[b] * Hg[b] * ug[b];
 }
 
-CAPAZ_HD void body_E(const DevParams& p, const DevFields& F, int64_t l) {
+template <class Lam>
+CAPAZ_HD void body_E_with(const DevParams& p, const DevFields& F, const Lam& lam, int64_t l) {
     int i, j, k;
     decompose(p, l, i, j, k);
     if (!interior3(p, i, j, k)) return;
     double T_flow = 0;
-    if (p.Nx > 2) T_flow += conduction(F, l, 1, p.hx2, p.r_hx2);
-    if (p.Ny > 2) T_flow += conduction(F, l, p.sy, p.hy2, p.r_hy2);
-    if (p.Nz > 2) T_flow = conduction(F, l, p.sz, p.hz2, p.r_hz2);  // '=' as energy.cpp:257
+    if (p.Nx > 2) T_flow += conduction(F, lam, l, 1, p.hx2, p.r_hx2);
+    if (p.Ny > 2) T_flow += conduction(F, lam, l, p.sy, p.hy2, p.r_hy2);
+    if (p.Nz > 2) T_flow = conduction(F, lam, l, p.sz, p.hz2, p.r_hz2);  // '=' as energy.cpp:257
     double E_flow = 0;
     if (p.Nx > 2)
         E_flow += exact_div(advection(F, F.f[F_ux_w], F.f[F_ux_n], F.f[F_ux_g], l + 1, l - 1), p.two_hx, p.r_two_hx);
@@ -219,40 +230,47 @@ CAPAZ_HD void body_E(const DevParams& p, const DevFields& F, int64_t l) {
     F.f[F_E_new][l] = F.f[F_E][l] + p.dt * (T_flow + Q_hw + Q_hr - E_flow);
 }
 
+CAPAZ_HD void body_E(const DevParams& p, const DevFields& F, int64_t l) { body_E_with(p, F, LamOfFields{F}, l); }
+
 // ---- P_S_calculation -> Newton (cpu.cpp:80-89), interior cells -------------------------
-CAPAZ_HD void body_newton(const DevParams& p, const DevFields& F, int64_t l) {
+// Reads the cell's state from R and writes the new state to W (the same
+// fields for the entry point; the second arrays of the state for the fused
+// energy step, whose neighbours still read the old state).
+CAPAZ_HD void body_newton_rw(const DevParams& p, const DevFields& R, const DevFields& W, int64_t l) {
     int i, j, k;
     decompose(p, l, i, j, k);
     if (p.model == 2) {
         if (!interior_newton2(p, i, j, k)) return;
-        double P = F.f[F_P_w][l], Sn = F.f[F_S_n][l];
-        newton2(p, P, Sn, F.f[F_roS_w][l], F.f[F_roS_n][l]);
-        F.f[F_P_w][l] = P;
-        F.f[F_S_n][l] = Sn;
+        double P = R.f[F_P_w][l], Sn = R.f[F_S_n][l];
+        newton2(p, P, Sn, R.f[F_roS_w][l], R.f[F_roS_n][l]);
+        W.f[F_P_w][l] = P;
+        W.f[F_S_n][l] = Sn;
         return;
     }
     if (!interior3(p, i, j, k)) return;
-    double P = F.f[F_P_w][l], Sw = F.f[F_S_w][l], Sn = F.f[F_S_n][l];
+    double P = R.f[F_P_w][l], Sw = R.f[F_S_w][l], Sn = R.f[F_S_n][l];
     if (p.energy) {
-        double Sg = F.f[F_S_g][l], T = F.f[F_T][l];
-        double Hw = F.f[F_H_w][l], Hn = F.f[F_H_n][l], Hg = F.f[F_H_g][l], Hr = F.f[F_H_r][l];
-        double E_new = F.f[F_E_new][l];
-        newton_energy(p, Sw, Sn, Sg, P, T, Hw, Hn, Hg, Hr, F.f[F_m][l], F.f[F_roS_w][l], F.f[F_roS_n][l],
-                      F.f[F_roS_g][l], E_new);
-        F.f[F_S_g][l] = Sg;
-        F.f[F_T][l] = T;
-        F.f[F_H_w][l] = Hw;
-        F.f[F_H_n][l] = Hn;
-        F.f[F_H_g][l] = Hg;
-        F.f[F_H_r][l] = Hr;
-        F.f[F_E][l] = E_new;
+        double Sg = R.f[F_S_g][l], T = R.f[F_T][l];
+        double Hw = R.f[F_H_w][l], Hn = R.f[F_H_n][l], Hg = R.f[F_H_g][l], Hr = R.f[F_H_r][l];
+        double E_new = R.f[F_E_new][l];
+        newton_energy(p, Sw, Sn, Sg, P, T, Hw, Hn, Hg, Hr, R.f[F_m][l], R.f[F_roS_w][l], R.f[F_roS_n][l],
+                      R.f[F_roS_g][l], E_new);
+        W.f[F_S_g][l] = Sg;
+        W.f[F_T][l] = T;
+        W.f[F_H_w][l] = Hw;
+        W.f[F_H_n][l] = Hn;
+        W.f[F_H_g][l] = Hg;
+        W.f[F_H_r][l] = Hr;
+        W.f[F_E][l] = E_new;
     } else {
-        newton3(p, P, Sw, Sn, F.f[F_roS_w][l], F.f[F_roS_n][l], F.f[F_roS_g][l]);
+        newton3(p, P, Sw, Sn, R.f[F_roS_w][l], R.f[F_roS_n][l], R.f[F_roS_g][l]);
     }
-    F.f[F_P_w][l] = P;
-    F.f[F_S_w][l] = Sw;
-    F.f[F_S_n][l] = Sn;
+    W.f[F_P_w][l] = P;
+    W.f[F_S_w][l] = Sw;
+    W.f[F_S_n][l] = Sn;
 }
+
+CAPAZ_HD void body_newton(const DevParams& p, const DevFields& F, int64_t l) { body_newton_rw(p, F, F, l); }
 
 // ---- boundary_conditions (cpu.cpp:91-104), boundary cells -------------------------------
 CAPAZ_HD void body_boundary(const DevParams& p, const DevFields& F, int64_t l) {
@@ -298,6 +316,27 @@ CAPAZ_HD void body_boundary(const DevParams& p, const DevFields& F, int64_t l) {
             T = F.f[F_T][l1];
         F.f[F_T][l] = T;
     }
+}
+
+// ---- fused energy step (three-phase + energy) ----------------------------------------------
+//
+// The entry-point sequence S, P_Xi_ro, u, roS, HE, E, Newton, boundary in six
+// passes, each cell's arithmetic that of the entry-point bodies above:
+//   A (all cells)      S, P_Xi_ro and HE of the cell (S_g, interior P_n/P_g,
+//                      Xi, ro, H, E in place), and the cell's lambda_eff into
+//                      a work array: E's conduction reads it at 7 points per
+//                      cell instead of evaluating it (a pow) 9 times;
+//   B (all cells)      u;   C (interior) roS;   E (interior) E_new;
+//   N (interior)       the Newton update;   D (boundary faces) the boundary.
+// The stencil passes stay apart: fused, their ~30 arrays read at 7 points
+// overflow L2 and run slower than apart, and the Newton update holds ~200
+// registers, which would cut the stencils' occupancy.
+
+CAPAZ_HD void body_energy_a(const DevParams& p, const DevFields& F, double* lam, int64_t l) {
+    body_S(p, F, l);
+    body_P_Xi_ro(p, F, l);
+    body_HE(p, F, l);
+    lam[l] = lambda_eff_at(F, l);
 }
 
 // ---- fused step (two- and three-phase without energy) -------------------------------------

@@ -22,6 +22,7 @@ struct capaz_ctx {
     bool m_has_zero = false;
     int* d_flag = nullptr;
     double* d_dtm = nullptr;  // dt / m per cell, refreshed with the porosity scan
+    double* d_lam = nullptr;  // energy model: lambda_eff per cell within a step
     capaz::Tuning tuning;
     // Event timing of the fused step's main kernel (capaz_tune "timing"):
     // a pool of event pairs, drained into the totals when it fills or is read.

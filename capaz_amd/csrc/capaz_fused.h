@@ -319,6 +319,16 @@ __global__ void __launch_bounds__(TX * TY) k_fused_z(DevParams p, StepIO io, int
                                                 FusedStore<MODEL>{io});
 }
 
+// Cell number t of a box of ni x nj cells per plane starting at (i0, j0, k0),
+// x fastest (32-bit index arithmetic: grids of < 2^31 cells).
+__device__ __forceinline__ void box_cell(uint32_t t, uint32_t ni, uint32_t nj, int i0, int j0, int k0, int& i, int& j,
+                                         int& k) {
+    const uint32_t a = t / ni, b = a / nj;
+    i = i0 + (int)(t - a * ni);
+    j = j0 + (int)(a - b * nj);
+    k = k0 + (int)b;
+}
+
 // Coordinates of boundary cell number t (t enumerates the j faces (all i,
 // k), then the x faces (j interior, all k), then the z faces (i, j
 // interior)); false for t past the last face cell.  32-bit index

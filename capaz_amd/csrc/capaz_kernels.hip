@@ -53,6 +53,22 @@ __global__ void __launch_bounds__(kBlock) k_fused_step(DevParams p, StepIO io) {
     if (l < p.n) body_fused_step(p, io, l);
 }
 
+// Energy step passes of their own: A and E (the rest are the entry-point
+// kernels), D over the boundary cells (face_cell's numbering).
+__global__ void __launch_bounds__(kBlock) k_energy_a(DevParams p, DevFields F, double* lam) {
+    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l < p.n) body_energy_a(p, F, lam, l);
+}
+__global__ void __launch_bounds__(kBlock) k_energy_e(DevParams p, DevFields F, const double* lam) {
+    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l < p.n) body_E_with(p, F, LamOfArray{lam}, l);
+}
+__global__ void __launch_bounds__(kBlock) k_energy_d(DevParams p, DevFields F) {
+    int i, j, k;
+    if (face_cell(p, (int64_t)blockIdx.x * kBlock + threadIdx.x, i, j, k))
+        body_boundary(p, F, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz);
+}
+
 // Copies the interior cells of src into dst (the three-phase P_n/P_g whose
 // interior values belong to the previous state after a fused step).
 __global__ void __launch_bounds__(kBlock) k_copy_interior(DevParams p, const double* src, double* dst) {
@@ -118,6 +134,21 @@ hipError_t launch_entry(Entry e, const DevParams& p, const DevFields& F, hipStre
         case Entry::Boundary: hipLaunchKernelGGL(k_boundary, g, b, 0, s, p, F); break;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* lam, hipStream_t s,
+                              const KernelEvents* ev) {
+    hipError_t e;
+    const dim3 g = grid_for(p.n), b(kBlock);
+    if (ev && (e = hipEventRecord(ev->begin, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_energy_a, g, b, 0, s, p, F, lam);
+    hipLaunchKernelGGL(k_u, g, b, 0, s, p, F);
+    hipLaunchKernelGGL(k_roS, g, b, 0, s, p, F);
+    hipLaunchKernelGGL(k_energy_e, g, b, 0, s, p, F, lam);
+    hipLaunchKernelGGL(k_newton, g, b, 0, s, p, F);
+    hipLaunchKernelGGL(k_energy_d, grid_for(fused_face_cells(p)), b, 0, s, p, F);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return ev ? hipEventRecord(ev->end, s) : hipSuccess;
 }
 
 hipError_t launch_fused_step(const DevParams& p, const StepIO& io, hipStream_t s) {

@@ -18,11 +18,15 @@ enum class Entry { S, P_Xi_ro, U, RoS, HE, E, Newton, Boundary };
 // recorded by the segment's forward recompute, then the coefficient pass per
 // cell; 1 = both passes on the z-marching LDS tiles; 0 = both per cell,
 // recomputing the neighbours' coefficients.
+// energy_kernel: 1 = the energy model's step in six passes (S, P_Xi_ro, HE
+// and lambda_eff per cell; u; roS; E from the stored lambda_eff; Newton;
+// boundary faces), 0 = the eight entry-point kernels.
 struct Tuning {
     int fused_kernel = 1;
     int tile_y = 8;
     int kchunk = 0;
     int adjoint_kernel = 2;
+    int energy_kernel = 1;
 };
 
 // Optional events recorded around the fused step's main kernel (the
@@ -36,6 +40,10 @@ hipError_t launch_entry(Entry e, const DevParams& p, const DevFields& F, hipStre
 hipError_t launch_fused_step(const DevParams& p, const StepIO& io, hipStream_t s);
 hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s,
                         const KernelEvents* ev = nullptr);
+// One step of the energy model in six passes, in place on F; lam is a work
+// array of Nx*Ny*Nz doubles; ev brackets the whole step.
+hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* lam, hipStream_t s,
+                              const KernelEvents* ev = nullptr);
 // Planes per tile of the fused z kernels: kchunk if >= 1, else chosen from
 // the grid and the number of workgroups resident at once, so that small grids
 // fill the GPU without a nearly empty last round.

@@ -168,8 +168,13 @@ int capaz_step(capaz_ctx* ctx, long first_step, long nsteps);
  *                   Newton from the flux divergence recorded while each
  *                   segment is recomputed; 1 both passes on z-marching LDS
  *                   tiles; 0 both one thread per cell
+ *   "energy_kernel" the energy model's step: 1 (default) six passes (per
+ *                   cell S, P_Xi_ro, HE and the conductivity; u; roS; E
+ *                   from the stored conductivity; Newton; the boundary
+ *                   faces); 0 the eight entry-point kernels
  *   "timing"        1: record HIP events around each fused step's main
- *                   kernel (read with capaz_kernel_time); 0: off (default).
+ *                   kernel (the energy model's whole step; read with
+ *                   capaz_kernel_time); 0: off (default).
  *                   Setting it resets the totals.
  * Returns CAPAZ_ERR_INVALID for an unknown key or value.  No counterpart in
  * the reference (gpu.cu's launch shape is fixed). */

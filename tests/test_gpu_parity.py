@@ -82,6 +82,10 @@ RANDOM_CASES = [
     ("3ph", (33, 30, 28), dict(DT=1e-4), False),
     ("2ph", (33, 30, 28), dict(DT=1e-4, SOURCE=4), False),
     ("3phe", (20, 18, 16), dict(DT=1e-4), False),
+    ("3phe", (1, 9, 1), dict(DT=1e-4), False),
+    ("3phe", (9, 8, 1), dict(DT=1e-4, SOURCE=2), False),
+    ("3phe", (1, 12, 9), dict(DT=1e-4), False),
+    ("3phe", (3, 3, 3), dict(DT=1e-4), False),
     ("3ph", (3, 3, 3), dict(CLOSURE="corey"), True),
     ("3ph", (1, 15, 1), dict(), False),
     ("3ph", (7, 3, 1), dict(SOURCE=2, CLOSURE="corey"), True),
@@ -160,6 +164,47 @@ def test_full_size_fused_equals_sequence():
     for f, v in pa.items():
         assert bit_equal(v, b.field(f)), f
     assert np.all(np.isfinite(pa["P_w"]))
+
+
+@pytest.mark.parametrize("dims", [(20, 18, 16), (9, 8, 1), (1, 12, 9), (300, 5, 4), (130, 3, 3)])
+def test_energy_passes_equal_the_sequence(dims):
+    keys = case("3phe", *dims, DT=1e-4, SOURCE=1)
+    st = random_state(keys, seed=sum(dims) + 11)
+    seq = capaz.Domain(keys, st)
+    seq.tune(energy_kernel=0)
+    seq.run(3)
+    d = capaz.Domain(keys, st)
+    d.run(3)
+    for f in seq.names:
+        assert bit_equal(d.field(f), seq.field(f)), f
+    seq.close()
+    d.close()
+
+
+def test_energy_full_size_six_passes_equal_the_sequence():
+    """The energy workload at 256 x 256 x 128: two steps of the six-pass
+    energy step and two entry-point sequences give identical bits in every
+    field."""
+    keys, st = workloads.energy_3ph(256, 256, 128)
+    a = capaz.Domain(keys, st)
+    assert a.tuning()["energy_kernel"] == 1
+    a.run(2)
+    pa = a.fields()
+    a.close()
+    b = capaz.Domain(keys, st)
+    b.time_step_function(keys["DT"])
+    b.time_step_function(2 * keys["DT"])
+    for f, v in pa.items():
+        assert bit_equal(v, b.field(f)), f
+    b.tune(energy_kernel=0)
+    b.run(1, first_step=3)
+    a = capaz.Domain(keys, pa)
+    a.run(1, first_step=3)
+    for f in ("P_w", "S_w", "S_n", "S_g", "T", "E", "H_w", "ux_g", "roS_g", "E_new"):
+        assert bit_equal(a.field(f), b.field(f)), f
+    assert np.all(np.isfinite(pa["T"]))
+    a.close()
+    b.close()
 
 
 def test_full_size_mass_conservation_without_gravity():
