@@ -347,7 +347,7 @@ int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
         for (long s = 0; s < nsteps; ++s) {
             const KernelEvents* ev;
             if ((rc = next_events(c, &ev))) return rc;
-            CAPAZ_HIP(c, launch_energy_step(c->p, fields_of(c), c->d_lam, c->stream, ev));
+            CAPAZ_HIP(c, launch_energy_step(c->p, fields_of(c), c->d_lam, c->d_dtm, c->stream, ev));
         }
         return CAPAZ_OK;
     }

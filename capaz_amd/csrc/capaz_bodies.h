@@ -355,6 +355,9 @@ struct StepIO {
     const double *Pw, *Sw, *Sn, *Pn, *Pg, *m, *K;  // current state (Sw unused for two phases)
     const double* dtm;                             // dt / m per cell (fused z kernel)
     double *Pw1, *Sw1, *Sn1, *Pn1, *Pg1;           // next state
+    // Xi and ro per phase as stored by P_Xi_ro (read by the fused z kernel's
+    // stored-coefficient variant only, the energy step's roS pass)
+    const double *Xi[3] = {}, *ro[3] = {};
 };
 
 // assign_P_Xi + assign_ro of one cell from the current primary state.

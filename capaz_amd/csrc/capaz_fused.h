@@ -46,9 +46,27 @@ __device__ __forceinline__ double corey_t(const DevParams& p, double s) {
 
 // assign_P_Xi + assign_ro of one cell with the model and closure fixed at
 // compile time (cpu.cpp:143-169; two-phase.cpp:4-23; three-phase.cpp:198-240).
+// CLOSURE 3 (three phases): the values the closure pass of this step has
+// already stored (P_n, P_g, Xi, ro arrays).
 template <int MODEL, int CLOSURE>
 __device__ __forceinline__ CoefT<MODEL> coef_at(const DevParams& p, const StepIO& io, int64_t l, bool inner) {
     CoefT<MODEL> c;
+    if constexpr (CLOSURE == 3) {
+        static_assert(MODEL == 3, "stored coefficients: three phases");
+        (void)p;
+        (void)inner;
+        c.Sw = io.Sw[l];
+        c.Sn = io.Sn[l];
+        c.P[0] = io.Pw[l];
+        c.P[1] = io.Pn[l];
+        c.P[2] = io.Pg[l];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            c.Xi[a] = io.Xi[a][l];
+            c.ro[a] = io.ro[a][l];
+        }
+        return c;
+    }
     const double Pw = io.Pw[l], K = io.K[l];
     if constexpr (MODEL == 2) {
         double kw, kn, Pk;
@@ -303,6 +321,30 @@ struct FusedStoreRecord {
         step(p, i, j, k, l, c, d);
     }
 };
+
+// The energy step's roS pass (assign_roS_nr, cpu.cpp:619-731) on the z tiles
+// from the stored coefficients (CLOSURE 3): roS_old and roS of the cell.
+struct EnergyRoSStore {
+    const double* dtm;
+    double *roS[3], *roS_old[3];
+    __device__ __forceinline__ void operator()(const DevParams& p, int i, int j, int k, int64_t l,
+                                               const CoefT<3>& c, const double* div) const {
+        (void)p; (void)i; (void)j; (void)k;
+        const double old[3] = {c.ro[0] * c.Sw, c.ro[1] * c.Sn, c.ro[2] * (1. - c.Sw - c.Sn)};
+        const double d = dtm[l];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            roS_old[a][l] = old[a];
+            roS[a][l] = old[a] - d * div[a];
+        }
+    }
+};
+
+template <int TX, int TY, bool CARRY>
+__global__ void __launch_bounds__(TX * TY) k_energy_roS_z(DevParams p, StepIO io, int kchunk, EnergyRoSStore st) {
+    __shared__ FusedLds<9, TX, TY> lds;
+    fused_z_tile<3, 3, TX, TY, CARRY>(p, io, kchunk, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, lds, st);
+}
 
 template <int MODEL, int CLOSURE, int TX, int TY, bool CARRY>
 __global__ void __launch_bounds__(TX * TY) k_fused_z_record(DevParams p, StepIO io, int kchunk, double* div,

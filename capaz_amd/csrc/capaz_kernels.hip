@@ -53,8 +53,9 @@ __global__ void __launch_bounds__(kBlock) k_fused_step(DevParams p, StepIO io) {
     if (l < p.n) body_fused_step(p, io, l);
 }
 
-// Energy step passes of their own: A and E (the rest are the entry-point
-// kernels), D over the boundary cells (face_cell's numbering).
+// Energy step passes of their own: A, E from the stored lambda_eff, and D
+// over the boundary cells (face_cell's numbering); roS runs on the z tiles
+// (capaz_fused.h), u and Newton are the entry-point kernels.
 __global__ void __launch_bounds__(kBlock) k_energy_a(DevParams p, DevFields F, double* lam) {
     int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (l < p.n) body_energy_a(p, F, lam, l);
@@ -134,21 +135,6 @@ hipError_t launch_entry(Entry e, const DevParams& p, const DevFields& F, hipStre
         case Entry::Boundary: hipLaunchKernelGGL(k_boundary, g, b, 0, s, p, F); break;
     }
     return hipGetLastError();
-}
-
-hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* lam, hipStream_t s,
-                              const KernelEvents* ev) {
-    hipError_t e;
-    const dim3 g = grid_for(p.n), b(kBlock);
-    if (ev && (e = hipEventRecord(ev->begin, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_energy_a, g, b, 0, s, p, F, lam);
-    hipLaunchKernelGGL(k_u, g, b, 0, s, p, F);
-    hipLaunchKernelGGL(k_roS, g, b, 0, s, p, F);
-    hipLaunchKernelGGL(k_energy_e, g, b, 0, s, p, F, lam);
-    hipLaunchKernelGGL(k_newton, g, b, 0, s, p, F);
-    hipLaunchKernelGGL(k_energy_d, grid_for(fused_face_cells(p)), b, 0, s, p, F);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    return ev ? hipEventRecord(ev->end, s) : hipSuccess;
 }
 
 hipError_t launch_fused_step(const DevParams& p, const StepIO& io, hipStream_t s) {
@@ -252,6 +238,46 @@ hipError_t launch_fused_record(const DevParams& p, const StepIO& io, int kchunk,
         hipLaunchKernelGGL((k_fused_faces<3, 1>), grid_for(fused_face_cells(p)), dim3(256), 0, s, p, io);
     }
     return hipGetLastError();
+}
+
+hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* lam, const double* dtm, hipStream_t s,
+                              const KernelEvents* ev) {
+    hipError_t e;
+    const dim3 g = grid_for(p.n), b(kBlock);
+    if (ev && (e = hipEventRecord(ev->begin, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_energy_a, g, b, 0, s, p, F, lam);
+    hipLaunchKernelGGL(k_u, g, b, 0, s, p, F);
+    {
+        // roS on the z tiles from the coefficients pass A stored
+        StepIO io{};
+        io.Pw = F.f[F_P_w];
+        io.Sw = F.f[F_S_w];
+        io.Sn = F.f[F_S_n];
+        io.Pn = F.f[F_P_n];
+        io.Pg = F.f[F_P_g];
+        io.m = F.f[F_m];
+        io.K = F.f[F_K];
+        io.dtm = dtm;
+        const int xi[3] = {F_Xi_w, F_Xi_n, F_Xi_g}, ro[3] = {F_ro_w, F_ro_n, F_ro_g};
+        const int rs[3] = {F_roS_w, F_roS_n, F_roS_g}, ro_old[3] = {F_roS_w_old, F_roS_n_old, F_roS_g_old};
+        EnergyRoSStore st{};
+        st.dtm = dtm;
+        for (int a = 0; a < 3; ++a) {
+            io.Xi[a] = F.f[xi[a]];
+            io.ro[a] = F.f[ro[a]];
+            st.roS[a] = F.f[rs[a]];
+            st.roS_old[a] = F.f[ro_old[a]];
+        }
+        const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
+        const int kchunk = fused_kchunk(p, 0, 8, resident_groups<k_energy_roS_z<64, 8, true>>(512));
+        dim3 grid((nx + 63) / 64, (ny + 7) / 8, (nz + kchunk - 1) / kchunk);
+        hipLaunchKernelGGL((k_energy_roS_z<64, 8, true>), grid, dim3(64, 8), 0, s, p, io, kchunk, st);
+    }
+    hipLaunchKernelGGL(k_energy_e, g, b, 0, s, p, F, lam);
+    hipLaunchKernelGGL(k_newton, g, b, 0, s, p, F);
+    hipLaunchKernelGGL(k_energy_d, grid_for(fused_face_cells(p)), b, 0, s, p, F);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return ev ? hipEventRecord(ev->end, s) : hipSuccess;
 }
 
 hipError_t launch_dtm(const DevParams& p, const double* m, double* dtm, hipStream_t s) {

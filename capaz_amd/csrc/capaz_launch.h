@@ -41,8 +41,9 @@ hipError_t launch_fused_step(const DevParams& p, const StepIO& io, hipStream_t s
 hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s,
                         const KernelEvents* ev = nullptr);
 // One step of the energy model in six passes, in place on F; lam is a work
-// array of Nx*Ny*Nz doubles; ev brackets the whole step.
-hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* lam, hipStream_t s,
+// array of Nx*Ny*Nz doubles, dtm holds dt / m per cell; ev brackets the whole
+// step.
+hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* lam, const double* dtm, hipStream_t s,
                               const KernelEvents* ev = nullptr);
 // Planes per tile of the fused z kernels: kchunk if >= 1, else chosen from
 // the grid and the number of workgroups resident at once, so that small grids
