@@ -23,6 +23,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "capaz_bodies.h"
 
 namespace capaz {
@@ -156,6 +158,14 @@ __device__ __forceinline__ void write_boundary(const DevParams& p, const StepIO&
     }
 }
 
+// An epilogue with kWantsU = true also receives the cell's Darcy velocities
+// u[phase][axis] (assign_u's central differences, cpu.cpp:346-477), from the
+// neighbours' pressures the tile already holds.
+template <class E, class = void>
+struct EpiWantsU : std::false_type {};
+template <class E>
+struct EpiWantsU<E, std::void_t<decltype(E::kWantsU)>> : std::bool_constant<E::kWantsU> {};
+
 template <int NV, int TX, int TY>
 struct FusedLds {
     double v[NV][TY + 2][TX + 2];
@@ -266,6 +276,21 @@ __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& i
                 double s = (a == 2 ? 0. + fx : fx) + fy;
                 div[a] = s + fz;
             }
+            if constexpr (EpiWantsU<Epi>::value) {
+                // interior cell: central differences on every axis longer than 2
+                double u[NPH][3];
+#pragma unroll
+                for (int a = 0; a < NPH; ++a) {
+                    u[a][0] = hasx ? c.Xi[a] * exact_div(lds.v[a][ty + 1][tx + 2] - lds.v[a][ty + 1][tx], p.two_hx,
+                                                         p.r_two_hx)
+                                   : 0.;
+                    u[a][1] = c.Xi[a] * (exact_div(lds.v[a][ty + 2][tx + 1] - lds.v[a][ty][tx + 1], p.two_hy,
+                                                   p.r_two_hy) -
+                                         c.ro[a] * p.g);
+                    u[a][2] = hasz ? c.Xi[a] * exact_div(zp.P[a] - zm.P[a], p.two_hz, p.r_two_hz) : 0.;
+                }
+                epi.store_u(l, u);
+            }
             epi(p, i, j, k, l, c, div);
         }
         __syncthreads();
@@ -275,6 +300,7 @@ __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& i
             dz[a] = dzp[a];
             zm.Xi[a] = c.Xi[a];
             zm.ro[a] = c.ro[a];
+            if constexpr (EpiWantsU<Epi>::value) zm.P[a] = c.P[a];
         }
         if (CARRY) c = zp;
     }
@@ -323,10 +349,19 @@ struct FusedStoreRecord {
 };
 
 // The energy step's roS pass (assign_roS_nr, cpu.cpp:619-731) on the z tiles
-// from the stored coefficients (CLOSURE 3): roS_old and roS of the cell.
+// from the stored coefficients (CLOSURE 3): roS_old and roS of the cell, and
+// its velocities (assign_u) from the same neighbours.
 struct EnergyRoSStore {
+    static constexpr bool kWantsU = true;
     const double* dtm;
     double *roS[3], *roS_old[3];
+    double* u[3][3];  // [phase][axis]: ux_w, uy_w, uz_w, ux_n, ...
+    __device__ __forceinline__ void store_u(int64_t l, const double (&v)[3][3]) const {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int x = 0; x < 3; ++x) u[a][x][l] = v[a][x];
+    }
     __device__ __forceinline__ void operator()(const DevParams& p, int i, int j, int k, int64_t l,
                                                const CoefT<3>& c, const double* div) const {
         (void)p; (void)i; (void)j; (void)k;

@@ -53,9 +53,10 @@ __global__ void __launch_bounds__(kBlock) k_fused_step(DevParams p, StepIO io) {
     if (l < p.n) body_fused_step(p, io, l);
 }
 
-// Energy step passes of their own: A, E from the stored lambda_eff, and D
-// over the boundary cells (face_cell's numbering); roS runs on the z tiles
-// (capaz_fused.h), u and Newton are the entry-point kernels.
+// Energy step passes of their own: A, E from the stored lambda_eff, u of the
+// boundary cells and D over the boundary cells (face_cell's numbering); roS
+// and the interior u run on the z tiles (capaz_fused.h), Newton is the entry
+// point.
 __global__ void __launch_bounds__(kBlock) k_energy_a(DevParams p, DevFields F, double* lam) {
     int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (l < p.n) body_energy_a(p, F, lam, l);
@@ -63,6 +64,11 @@ __global__ void __launch_bounds__(kBlock) k_energy_a(DevParams p, DevFields F, d
 __global__ void __launch_bounds__(kBlock) k_energy_e(DevParams p, DevFields F, const double* lam) {
     int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (l < p.n) body_E_with(p, F, LamOfArray{lam}, l);
+}
+__global__ void __launch_bounds__(kBlock) k_energy_u_faces(DevParams p, DevFields F) {
+    int i, j, k;
+    if (face_cell(p, (int64_t)blockIdx.x * kBlock + threadIdx.x, i, j, k))
+        body_u(p, F, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz);
 }
 __global__ void __launch_bounds__(kBlock) k_energy_d(DevParams p, DevFields F) {
     int i, j, k;
@@ -246,7 +252,8 @@ hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* la
     const dim3 g = grid_for(p.n), b(kBlock);
     if (ev && (e = hipEventRecord(ev->begin, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_energy_a, g, b, 0, s, p, F, lam);
-    hipLaunchKernelGGL(k_u, g, b, 0, s, p, F);
+    // u of the boundary cells here, of the interior cells in the roS tiles
+    hipLaunchKernelGGL(k_energy_u_faces, grid_for(fused_face_cells(p)), b, 0, s, p, F);
     {
         // roS on the z tiles from the coefficients pass A stored
         StepIO io{};
@@ -260,6 +267,7 @@ hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* la
         io.dtm = dtm;
         const int xi[3] = {F_Xi_w, F_Xi_n, F_Xi_g}, ro[3] = {F_ro_w, F_ro_n, F_ro_g};
         const int rs[3] = {F_roS_w, F_roS_n, F_roS_g}, ro_old[3] = {F_roS_w_old, F_roS_n_old, F_roS_g_old};
+        const int uf[3][3] = {{F_ux_w, F_uy_w, F_uz_w}, {F_ux_n, F_uy_n, F_uz_n}, {F_ux_g, F_uy_g, F_uz_g}};
         EnergyRoSStore st{};
         st.dtm = dtm;
         for (int a = 0; a < 3; ++a) {
@@ -267,6 +275,7 @@ hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* la
             io.ro[a] = F.f[ro[a]];
             st.roS[a] = F.f[rs[a]];
             st.roS_old[a] = F.f[ro_old[a]];
+            for (int x = 0; x < 3; ++x) st.u[a][x] = F.f[uf[a][x]];
         }
         const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
         const int kchunk = fused_kchunk(p, 0, 8, resident_groups<k_energy_roS_z<64, 8, true>>(512));
