@@ -136,7 +136,8 @@ def bench_adjoint(n: int = 96, nsteps: int = 400, directions: int = 20) -> dict:
     history_matching: three-phase n^3, nsteps steps, 16 observation columns
     sampled every 50 steps, observations from a seeded true ln K, trial ln K
     at its mean): wall time of the objective alone (capaz_objective) and of
-    objective + gradient (capaz_adjoint, sqrt(N) checkpoints), and central
+    objective + gradient (capaz_adjoint with its default segment choice, and
+    with sqrt(N) checkpoints), and central
     differences of capaz_objective along random directions in (ln K, m),
     each also against the tangent-linear model's exact directional derivative
     (capaz_tangent)."""
@@ -173,13 +174,16 @@ def bench_adjoint(n: int = 96, nsteps: int = 400, directions: int = 20) -> dict:
         return out, best
 
     J, t_fwd = timed(lambda: d.objective(nsteps, obs))
+    # default segment choice: the whole trajectory recorded by the forward
+    # sweep when it fits in a quarter of the device memory (no recompute)
     (J_a, gK, gm), t_adj = timed(lambda: d.adjoint(nsteps, obs))
-    # one segment: the whole trajectory recorded by the forward sweep, no recompute
-    (_, gK1, gm1), t_adj1 = timed(lambda: d.adjoint(nsteps, obs, checkpoint_every=nsteps))
+    segment = d.tuning()["adjoint_segment"]
+    # sqrt(N) checkpoints, each segment recomputed on the way back
+    (_, gK1, gm1), t_adj1 = timed(lambda: d.adjoint(nsteps, obs, checkpoint_every=-1))
     d.tune(adjoint_kernel=0)
-    _, t_adj_cell = timed(lambda: d.adjoint(nsteps, obs))
+    _, t_adj_cell = timed(lambda: d.adjoint(nsteps, obs, checkpoint_every=-1))
     d.tune(adjoint_kernel=1)
-    _, t_adj_tiled = timed(lambda: d.adjoint(nsteps, obs))
+    _, t_adj_tiled = timed(lambda: d.adjoint(nsteps, obs, checkpoint_every=-1))
     d.tune(adjoint_kernel=2)
     every = int(round(nsteps ** 0.5))
     rng = np.random.default_rng(2024)
@@ -207,11 +211,11 @@ def bench_adjoint(n: int = 96, nsteps: int = 400, directions: int = 20) -> dict:
                         f"gradient in per-cell ln K and porosity",
             "J": J, "J_equal": J == J_a,
             "forward_ms": round(t_fwd * 1e3, 2), "adjoint_ms": round(t_adj * 1e3, 2),
-            "adjoint_over_forward": round(t_adj / t_fwd, 2), "checkpoint_every": every,
-            "adjoint_one_segment_ms": round(t_adj1 * 1e3, 2),
-            "adjoint_recompute_stencil_per_cell_ms": round(t_adj_cell * 1e3, 2),
-            "adjoint_recompute_stencil_tiled_ms": round(t_adj_tiled * 1e3, 2),
-            "adjoint_one_segment_over_forward": round(t_adj1 / t_fwd, 2),
+            "adjoint_over_forward": round(t_adj / t_fwd, 2), "adjoint_segment": segment,
+            "adjoint_sqrt_n_ms": round(t_adj1 * 1e3, 2), "sqrt_n_checkpoint_every": every,
+            "adjoint_sqrt_n_over_forward": round(t_adj1 / t_fwd, 2),
+            "adjoint_sqrt_n_recompute_stencil_per_cell_ms": round(t_adj_cell * 1e3, 2),
+            "adjoint_sqrt_n_recompute_stencil_tiled_ms": round(t_adj_tiled * 1e3, 2),
             "checkpointing_changes_gradient": bool(not (np.array_equal(gK, gK1) and np.array_equal(gm, gm1))),
             "tangent": {"ms": round(t_tan * 1e3, 2),
                         "max_rel_err_adjoint_vs_tangent": float(max(tan_errs)),

@@ -259,7 +259,8 @@ class Domain:
 
     def tuning(self) -> dict:
         out = {}
-        for k in ("fused_kernel", "tile_y", "kchunk", "adjoint_kernel", "energy_kernel", "timing"):
+        for k in ("fused_kernel", "tile_y", "kchunk", "adjoint_kernel", "energy_kernel", "timing",
+                  "adjoint_segment"):
             v = ctypes.c_int()
             self._check(load().capaz_get_tuning(self._ctx, k.encode(), ctypes.byref(v)), "capaz_get_tuning")
             out[k] = v.value
@@ -284,7 +285,11 @@ class Domain:
         return J.value
 
     def adjoint(self, nsteps: int, obs: dict, checkpoint_every: int = 0):
-        """capaz_adjoint: (J, dJ/d lnK, dJ/dm) after nsteps fused steps from the current state."""
+        """capaz_adjoint: (J, dJ/d lnK, dJ/dm) after nsteps fused steps from the
+        current state.  checkpoint_every: > 0 the segment length, < 0
+        round(sqrt(nsteps)), 0 the whole trajectory when it fits in a quarter
+        of the device memory, else round(sqrt(nsteps)) (tuning()
+        ["adjoint_segment"] reports the one used)."""
         o, keep = make_observations(obs["cells"], obs["steps"], obs["S_w"], obs["S_n"])
         J = ctypes.c_double()
         gK, gm = np.empty(self.shape), np.empty(self.shape)

@@ -2,11 +2,12 @@
 // (include/capaz.h: capaz_objective, capaz_adjoint).
 //
 // capaz_adjoint runs the forward steps from the context's state, keeping a
-// checkpoint of the primary state every C steps (C = round(sqrt(nsteps)) by
-// default) and the observed saturations at the sampled steps, then sweeps the
-// segments backwards: each segment is recomputed from its checkpoint with
-// the fused kernel (the same bits as the first pass), recording each cell's
-// flux divergence, and its steps are reversed with the two adjoint passes.
+// checkpoint of the primary state every C steps and the observed saturations
+// at the sampled steps, then sweeps the segments backwards: each segment is
+// recomputed from its checkpoint with the fused kernel (the same bits as the
+// first pass), recording each cell's flux divergence, and its steps are
+// reversed with the two adjoint passes.  The last segment is recorded by the
+// forward sweep itself, so with C = nsteps nothing is recomputed.
 // Device memory: (nsteps/C + C + 1) states of five fields, 3C recorded
 // fields and 18 fields of adjoint work, held by the context between calls.
 #include <hip/hip_runtime.h>
@@ -303,21 +304,48 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
     if ((rc = check_obs(c, nsteps, o, &ob))) return rc;
     if ((rc = detail::scan_m(c))) return rc;
     if (c->m_has_zero) return fail(c, CAPAZ_ERR_STATE, "adjoint: cells with m == 0");
-    long C = checkpoint_every > 0 ? checkpoint_every : std::lround(std::sqrt((double)nsteps));
-    C = std::max(1L, std::min(C, nsteps));
-    const long nseg = (nsteps + C - 1) / C;
     const size_t S = field_stride(c->p.n);
     const size_t n_state = (size_t)kState * S;
     // layout: checkpoints | segment states (C + 1) | recorded divergence (C x 3) | lambda x2 | work (6) |
     // grads (2) | seen | observed | cells
     const bool recorded = c->tuning.adjoint_kernel == 2;
     const size_t ns = (size_t)ob.nsamples * 2 * ob.ncells;
-    const size_t n_div = recorded ? (size_t)C * 3 * S : 0;
     const size_t n_work = 6 * S;
-    const size_t total = (size_t)nseg * n_state + (size_t)(C + 1) * n_state + n_div + 2 * n_state + n_work + 2 * S +
-                         2 * ns + (size_t)ob.ncells;
+    auto doubles_for = [&](long seg) {
+        const long segs = (nsteps + seg - 1) / seg;
+        return (size_t)segs * n_state + (size_t)(seg + 1) * n_state + (recorded ? (size_t)seg * 3 * S : 0) +
+               2 * n_state + n_work + 2 * S + 2 * ns + (size_t)ob.ncells;
+    };
+    // Segment length C: checkpoint_every if > 0; round(sqrt(nsteps)) if < 0;
+    // if 0, the whole trajectory as one segment (recorded by the forward
+    // sweep, nothing recomputed) when that takes at most a quarter of the
+    // device's memory and fits in what is free, else round(sqrt(nsteps)).
+    // Every C gives the same bits.
+    const long sqrt_c = std::max(1L, std::min((long)std::lround(std::sqrt((double)nsteps)), nsteps));
+    long C = checkpoint_every > 0 ? std::max(1L, std::min((long)checkpoint_every, nsteps)) : sqrt_c;
+    bool whole = false;
+    if (checkpoint_every == 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) {
+            const size_t need = doubles_for(nsteps) * sizeof(double);
+            whole = need <= total_b / 4 && need + ((size_t)1 << 30) <= free_b + c->adj_bytes;
+        } else {
+            (void)hipGetLastError();
+        }
+        if (whole) C = nsteps;
+    }
     double* w;
-    if ((rc = workspace(c, total, &w))) return rc;
+    rc = workspace(c, doubles_for(C), &w);
+    if (rc == CAPAZ_ERR_NOMEM && whole) {
+        // fall back to sqrt(N) checkpoints
+        (void)hipGetLastError();
+        C = sqrt_c;
+        rc = workspace(c, doubles_for(C), &w);
+    }
+    if (rc) return rc;
+    c->adj_segment = C;
+    const long nseg = (nsteps + C - 1) / C;
+    const size_t n_div = recorded ? (size_t)C * 3 * S : 0;
     double* ckpt = w;
     double* seg = ckpt + (size_t)nseg * n_state;
     double* divs = seg + (size_t)(C + 1) * n_state;

@@ -13,6 +13,7 @@
 // The energy model's step (six passes) works in place like the sequence.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -416,6 +417,7 @@ int capaz_get_tuning(const capaz_ctx* c, const char* key, int* value) {
     else if (k == "adjoint_kernel") *value = c->tuning.adjoint_kernel;
     else if (k == "energy_kernel") *value = c->tuning.energy_kernel;
     else if (k == "timing") *value = c->timing ? 1 : 0;
+    else if (k == "adjoint_segment") *value = (int)std::min<long>(c->adj_segment, 0x7fffffff);
     else return CAPAZ_ERR_INVALID;
     return CAPAZ_OK;
 }

@@ -34,6 +34,7 @@ struct capaz_ctx {
     // adjoint workspace (checkpoints, segment states, adjoint fields)
     double* adj_mem = nullptr;
     size_t adj_bytes = 0;
+    long adj_segment = 0;  // segment length C of the last capaz_adjoint
     std::string err;
 };
 

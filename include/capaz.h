@@ -175,6 +175,8 @@ int capaz_step(capaz_ctx* ctx, long first_step, long nsteps);
  *   "timing"        1: record HIP events around each fused step's main
  *                   kernel (the energy model's whole step; read with
  *                   capaz_kernel_time); 0: off (default).
+ *   "adjoint_segment" (read only) the segment length of the last
+ *                   capaz_adjoint
  *                   Setting it resets the totals.
  * Returns CAPAZ_ERR_INVALID for an unknown key or value.  No counterpart in
  * the reference (gpu.cu's launch shape is fixed). */
@@ -233,10 +235,15 @@ typedef struct capaz_observations {
 int capaz_objective(capaz_ctx* ctx, long nsteps, const capaz_observations* obs, double* J);
 /* J and its gradient with respect to ln K and m of every cell (host arrays
  * of Nx*Ny*Nz doubles, either may be NULL), by the hand-written discrete
- * adjoint of the fused step, with a checkpoint of the state every
- * checkpoint_every steps (<= 0: round(sqrt(nsteps))) and each segment
- * recomputed from its checkpoint on the way back.  The context then holds the
- * state after nsteps, as after capaz_objective. */
+ * adjoint of the fused step, with a checkpoint of the state every C steps and
+ * each segment recomputed from its checkpoint on the way back.
+ * C = checkpoint_every if > 0; round(sqrt(nsteps)) if < 0; if 0, nsteps (the
+ * forward sweep records the whole trajectory and nothing is recomputed) when
+ * that workspace takes at most a quarter of the device's memory and is free,
+ * else round(sqrt(nsteps)).  The gradient has the same bits for every C; the
+ * C used is read back with capaz_get_tuning(ctx, "adjoint_segment").  The
+ * workspace stays with the context for the next call.  The context then
+ * holds the state after nsteps, as after capaz_objective. */
 int capaz_adjoint(capaz_ctx* ctx, long nsteps, const capaz_observations* obs, int checkpoint_every, double* J,
                   double* grad_lnK, double* grad_m);
 
