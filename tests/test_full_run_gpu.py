@@ -1,0 +1,51 @@
+"""Whole runs against the oracle on device 0: the configs' step counts on
+grids the oracle finishes in seconds.
+
+  * two-phase Buckley-Leverett at 40^3, 1000 steps (the config as specified)
+    and the three-phase injection physics at 48^3, 2000 steps (the 128^3
+    config's step count): every field bit-identical to the oracle after the
+    whole run (Corey closures);
+  * the three-phase + energy model (reference deck and closures, whose pow
+    is the device library's, not glibc's) at 24 x 20 x 16, 300 steps: every
+    field within 1e-9 of the oracle relative to the field's magnitude, and
+    P_w, S_w, S_n element by element -- the full-run bar.
+"""
+import numpy as np
+import pytest
+
+from capaz_amd import capaz, workloads
+from capaz_cases import bit_equal, max_rel, norm_rel
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("P_w", "S_w", "S_n")
+
+
+def run_both(keys, st, nsteps):
+    od = oracle.OracleDomain(keys, st)
+    od.run(nsteps)
+    d = capaz.Domain(keys, st)
+    d.run(nsteps)
+    got = d.fields()
+    d.close()
+    return got, od.fields()
+
+
+@pytest.mark.parametrize("name,nsteps", [("buckley_leverett_40", 1000), ("three_phase_48", 2000)])
+def test_whole_run_is_bit_identical(name, nsteps):
+    keys, st = workloads.buckley_leverett_2ph(40) if name.startswith("buckley") else \
+        workloads.three_phase_injection(48)
+    got, want = run_both(keys, st, nsteps)
+    assert np.isfinite(got["P_w"]).all()
+    for f, v in want.items():
+        assert bit_equal(got[f], v), (f, max_rel(got[f], v))
+
+
+def test_energy_whole_run_within_1e9():
+    keys, st = workloads.energy_3ph(24, 20, 16)
+    got, want = run_both(keys, st, 300)
+    for f, v in want.items():
+        assert norm_rel(got[f], v) <= 1e-9, (f, norm_rel(got[f], v))
+        if f in STATE:
+            assert max_rel(got[f], v) <= 1e-9, (f, max_rel(got[f], v))
