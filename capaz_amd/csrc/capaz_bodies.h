@@ -233,44 +233,41 @@ CAPAZ_HD void body_E_with(const DevParams& p, const DevFields& F, const Lam& lam
 CAPAZ_HD void body_E(const DevParams& p, const DevFields& F, int64_t l) { body_E_with(p, F, LamOfFields{F}, l); }
 
 // ---- P_S_calculation -> Newton (cpu.cpp:80-89), interior cells -------------------------
-// Reads the cell's state from R and writes the new state to W (the same
-// fields for the entry point; the second arrays of the state for the fused
-// energy step, whose neighbours still read the old state).
-CAPAZ_HD void body_newton_rw(const DevParams& p, const DevFields& R, const DevFields& W, int64_t l) {
+CAPAZ_HD void body_newton(const DevParams& p, const DevFields& F, int64_t l) {
     int i, j, k;
     decompose(p, l, i, j, k);
     if (p.model == 2) {
         if (!interior_newton2(p, i, j, k)) return;
-        double P = R.f[F_P_w][l], Sn = R.f[F_S_n][l];
-        newton2(p, P, Sn, R.f[F_roS_w][l], R.f[F_roS_n][l]);
-        W.f[F_P_w][l] = P;
-        W.f[F_S_n][l] = Sn;
+        double P = F.f[F_P_w][l], Sn = F.f[F_S_n][l];
+        newton2(p, P, Sn, F.f[F_roS_w][l], F.f[F_roS_n][l]);
+        F.f[F_P_w][l] = P;
+        F.f[F_S_n][l] = Sn;
         return;
     }
     if (!interior3(p, i, j, k)) return;
-    double P = R.f[F_P_w][l], Sw = R.f[F_S_w][l], Sn = R.f[F_S_n][l];
+    double P = F.f[F_P_w][l], Sw = F.f[F_S_w][l], Sn = F.f[F_S_n][l];
     if (p.energy) {
-        double Sg = R.f[F_S_g][l], T = R.f[F_T][l];
-        double Hw = R.f[F_H_w][l], Hn = R.f[F_H_n][l], Hg = R.f[F_H_g][l], Hr = R.f[F_H_r][l];
-        double E_new = R.f[F_E_new][l];
-        newton_energy(p, Sw, Sn, Sg, P, T, Hw, Hn, Hg, Hr, R.f[F_m][l], R.f[F_roS_w][l], R.f[F_roS_n][l],
-                      R.f[F_roS_g][l], E_new);
-        W.f[F_S_g][l] = Sg;
-        W.f[F_T][l] = T;
-        W.f[F_H_w][l] = Hw;
-        W.f[F_H_n][l] = Hn;
-        W.f[F_H_g][l] = Hg;
-        W.f[F_H_r][l] = Hr;
-        W.f[F_E][l] = E_new;
+        double Sg = F.f[F_S_g][l], T = F.f[F_T][l];
+        double Hw = F.f[F_H_w][l], Hn = F.f[F_H_n][l], Hg = F.f[F_H_g][l], Hr = F.f[F_H_r][l];
+        double E_new = F.f[F_E_new][l];
+        newton_energy(p, Sw, Sn, Sg, P, T, Hw, Hn, Hg, Hr, F.f[F_m][l], F.f[F_roS_w][l], F.f[F_roS_n][l],
+                      F.f[F_roS_g][l], E_new);
+        F.f[F_S_g][l] = Sg;
+        F.f[F_T][l] = T;
+        F.f[F_H_w][l] = Hw;
+        F.f[F_H_n][l] = Hn;
+        F.f[F_H_g][l] = Hg;
+        F.f[F_H_r][l] = Hr;
+        F.f[F_E][l] = E_new;
     } else {
-        newton3(p, P, Sw, Sn, R.f[F_roS_w][l], R.f[F_roS_n][l], R.f[F_roS_g][l]);
+        newton3(p, P, Sw, Sn, F.f[F_roS_w][l], F.f[F_roS_n][l], F.f[F_roS_g][l]);
     }
-    W.f[F_P_w][l] = P;
-    W.f[F_S_w][l] = Sw;
-    W.f[F_S_n][l] = Sn;
+    F.f[F_P_w][l] = P;
+    F.f[F_S_w][l] = Sw;
+    F.f[F_S_n][l] = Sn;
 }
 
-CAPAZ_HD void body_newton(const DevParams& p, const DevFields& F, int64_t l) { body_newton_rw(p, F, F, l); }
+
 
 // ---- boundary_conditions (cpu.cpp:91-104), boundary cells -------------------------------
 CAPAZ_HD void body_boundary(const DevParams& p, const DevFields& F, int64_t l) {
