@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(TX * TY)
         const int lt = t - d->tile_begin;
         const int bx = lt % d->tiles_x, r = lt / d->tiles_x;
         fused_z_tile<MODEL, CLOSURE, TX, TY, true>(d->p, d->io[parity], kchunk, bx, r % d->tiles_y, r / d->tiles_y,
-                                                   lds, FusedStore<MODEL>{d->io[parity]});
+                                                   lds, FusedStore<MODEL, kFusedFaces<MODEL>>{d->io[parity]});
         if (!counter) return;
     }
 }
@@ -65,7 +65,8 @@ hipError_t step_shape(const BatchShape& sh, const BatchDomain* doms, const int* 
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    if (nface_blocks > 0)
+    // two phases: the tiles write the boundary cells themselves
+    if (nface_blocks > 0 && !kFusedFaces<MODEL>)
         hipLaunchKernelGGL((k_batch_faces<MODEL, CLOSURE>), dim3(nface_blocks), dim3(256), 0, s, doms, face_owner,
                            parity);
     return hipGetLastError();

@@ -306,8 +306,47 @@ __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& i
     }
 }
 
-// The step's epilogue: A = roS - dt/m * div, the Newton solve, the new state.
+// The boundary cells whose inward neighbour (set_boundary_basic_coordinate,
+// cpu.cpp:107-141) is the interior cell (i, j, k), from its new state: the
+// cell's offsets along each axis are 0 and -1 next to the low face, +1 next to
+// the high face.  On the supported grids every boundary cell has exactly one
+// interior inward neighbour, so this writes each boundary cell once, with the
+// values k_fused_faces would write.
 template <int MODEL>
+__device__ __forceinline__ void store_faces(const DevParams& p, const StepIO& io, int i, int j, int k, int64_t l,
+                                            const CoefT<MODEL>& c, double Pw, double Sw, double Sn) {
+    const bool hasx = p.Nx >= 3, hasz = p.Nz >= 3;
+    const bool xl = hasx && i == 1, xh = hasx && i == p.Nx - 2;
+    const bool yl = j == 1, yh = j == p.Ny - 2;
+    const bool zl = hasz && k == 1, zh = hasz && k == p.Nz - 2;
+    if (!(xl || xh || yl || yh || zl || zh)) return;
+    double ro_eff = 0.;
+    if constexpr (MODEL == 2) {
+        // ro_eff_gdy (main.cpp:255-267) for the bottom face: this step's
+        // densities of the cell (from its previous state), its new saturation
+        if (yh) ro_eff = (c.ro[1] * Sn + c.ro[0] * (1 - Sn)) * io.m[l] * p.g * p.hy;
+    }
+    int xo[3] = {0, 0, 0}, yo[3] = {0, 0, 0}, zo[3] = {0, 0, 0};
+    int nx = 1, ny = 1, nz = 1;
+    if (xl) xo[nx++] = -1;
+    if (xh) xo[nx++] = 1;
+    if (yl) yo[ny++] = -1;
+    if (yh) yo[ny++] = 1;
+    if (zl) zo[nz++] = -1;
+    if (zh) zo[nz++] = 1;
+    for (int a = 0; a < nx; ++a)
+        for (int b = 0; b < ny; ++b)
+            for (int d = 0; d < nz; ++d)
+                if (a || b || d) write_boundary<MODEL>(p, io, i + xo[a], j + yo[b], k + zo[d], Pw, Sw, Sn, ro_eff);
+}
+
+// The step's epilogue: A = roS - dt/m * div, the Newton solve, the new state;
+// FACES: also the boundary cells this cell is the inward neighbour of (else
+// k_fused_faces writes them after the kernel).  The two-phase kernels write
+// their faces here (the strided x faces cost a separate pass more than the
+// epilogue costs); in the three-phase kernels the extra registers would cost
+// a wave per SIMD, so they keep the face pass.
+template <int MODEL, bool FACES = false>
 struct FusedStore {
     StepIO io;
     __device__ __forceinline__ void operator()(const DevParams& p, int i, int j, int k, int64_t l,
@@ -321,6 +360,7 @@ struct FusedStore {
             newton2(p, Pw, Sn, A1, A2);
             io.Pw1[l] = Pw;
             io.Sn1[l] = Sn;
+            if constexpr (FACES) store_faces<2>(p, io, i, j, k, l, c, Pw, 0., Sn);
         } else {
             const double A1 = c.ro[0] * Sw - dtm * div[0];
             const double A2 = c.ro[1] * Sn - dtm * div[1];
@@ -329,6 +369,7 @@ struct FusedStore {
             io.Pw1[l] = Pw;
             io.Sw1[l] = Sw;
             io.Sn1[l] = Sn;
+            if constexpr (FACES) store_faces<3>(p, io, i, j, k, l, c, Pw, Sw, Sn);
         }
     }
 };
@@ -389,11 +430,16 @@ __global__ void __launch_bounds__(TX * TY) k_fused_z_record(DevParams p, StepIO 
                                                 FusedStoreRecord<MODEL>{FusedStore<MODEL>{io}, div, stride});
 }
 
+// Two phases: boundary cells included (kFusedFaces); three phases: followed
+// by k_fused_faces.
+template <int MODEL>
+constexpr bool kFusedFaces = MODEL == 2;
+
 template <int MODEL, int CLOSURE, int TX, int TY, bool CARRY>
 __global__ void __launch_bounds__(TX * TY) k_fused_z(DevParams p, StepIO io, int kchunk) {
     __shared__ FusedLds<3 * MODEL, TX, TY> lds;
     fused_z_tile<MODEL, CLOSURE, TX, TY, CARRY>(p, io, kchunk, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, lds,
-                                                FusedStore<MODEL>{io});
+                                                FusedStore<MODEL, kFusedFaces<MODEL>>{io});
 }
 
 // Cell number t of a box of ni x nj cells per plane starting at (i0, j0, k0),

@@ -193,7 +193,8 @@ hipError_t launch_z(const DevParams& p, const StepIO& io, int kchunk, hipStream_
     hipLaunchKernelGGL((k_fused_z<MODEL, CLOSURE, 64, TY, CARRY>), grid, dim3(64, TY), 0, s, p, io, kchunk);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev->end, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL((k_fused_faces<MODEL, CLOSURE>), grid_for(fused_face_cells(p)), dim3(256), 0, s, p, io);
+    if constexpr (!kFusedFaces<MODEL>)
+        hipLaunchKernelGGL((k_fused_faces<MODEL, CLOSURE>), grid_for(fused_face_cells(p)), dim3(256), 0, s, p, io);
     return hipGetLastError();
 }
 
