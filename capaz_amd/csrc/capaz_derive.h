@@ -19,7 +19,6 @@ inline std::string validate(const capaz_params& u) {
     if (u.energy != 0 && u.energy != 1) return "energy must be 0 or 1";
     if (u.energy && u.model != 3) return "the energy equation exists for the three-phase model only (energy.cpp)";
     if (u.closure != 0 && u.closure != 1) return "closure must be CAPAZ_CLOSURE_REFERENCE or CAPAZ_CLOSURE_COREY";
-    if (u.energy && u.closure != 0) return "the energy model supports the reference closure only";
     if (u.Ny < 3) return "Ny must be >= 3";
     if (!(u.Nx == 1 || u.Nx >= 3)) return "Nx must be 1 or >= 3";
     if (u.model == 2 && u.Nx < 3) return "the two-phase model needs Nx >= 3 (two-phase.cpp:131,165)";

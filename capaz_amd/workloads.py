@@ -157,16 +157,20 @@ REF_3PH_DECK = dict(
 )
 
 
-def energy_3ph(nx: int = 256, ny: int = 256, nz: int = 128, seed: int = 1002):
+def energy_3ph(nx: int = 256, ny: int = 256, nz: int = 128, seed: int = 1002, closure: str = "reference"):
     """The reference's three-phase model with the energy equation
     (-DTHREE_PHASE -DENERGY; its Three-phase/defines.ini deck, reference
     closures, 3 Newton iterations, dt = 1e-4 s) on an nx x ny x nz grid.
     Initial state: hydrostatic water pressure, S = (0.4, 0.3, 0.3), T = 285 K
     (three-phase.cpp:470-487), porosity 0.4(1 + 0.05 N(0,1)) clipped to
     [0.3, 0.5], uniform K_0.  The reference has no two-phase energy model, so
-    this is its own energy model on the grid of the non-isothermal config."""
+    this is its own energy model on the grid of the non-isothermal config.
+    closure="corey": the relative permeabilities of the two-phase workload
+    instead (Corey-2, S_wr 0.1, S_nr 0.05), the rest unchanged."""
     rng = np.random.default_rng(seed)
     keys = dict(REF_3PH_DECK, ENERGY=1, ITERATIONS=3, DT=1e-4, NX=nx, NY=ny, NZ=nz)
+    if closure == "corey":
+        keys.update(CLOSURE="corey", COREY_N=2, S_WR_0=0.1, S_NR_0=0.05)
     shape = (nz, ny, nx)
     j = np.arange(ny).reshape(1, ny, 1)
     P = np.broadcast_to(keys["P_ATM"] + 1000.0 * keys["G_CONST"] * keys["HY"] * j, shape).copy()

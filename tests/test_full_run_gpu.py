@@ -5,8 +5,9 @@ grids the oracle finishes in seconds.
     and the three-phase injection physics at 48^3, 2000 steps (the 128^3
     config's step count): every field bit-identical to the oracle after the
     whole run (Corey closures);
-  * the three-phase + energy model (reference deck and closures, whose pow
-    is the device library's, not glibc's) at 24 x 20 x 16, 300 steps: every
+  * the three-phase + energy model (reference deck, its closures or Corey-2;
+    mu(T) and lambda_g(T) call exp/pow, the device library's, not glibc's)
+    at 24 x 20 x 16, 300 steps: every
     field within 1e-9 of the oracle relative to the field's magnitude, and
     P_w, S_w, S_n element by element -- the full-run bar.
 """
@@ -42,8 +43,9 @@ def test_whole_run_is_bit_identical(name, nsteps):
         assert bit_equal(got[f], v), (f, max_rel(got[f], v))
 
 
-def test_energy_whole_run_within_1e9():
-    keys, st = workloads.energy_3ph(24, 20, 16)
+@pytest.mark.parametrize("closure", ["reference", "corey"])
+def test_energy_whole_run_within_1e9(closure):
+    keys, st = workloads.energy_3ph(24, 20, 16, closure=closure)
     got, want = run_both(keys, st, 300)
     for f, v in want.items():
         assert norm_rel(got[f], v) <= 1e-9, (f, norm_rel(got[f], v))

@@ -86,6 +86,7 @@ RANDOM_CASES = [
     ("3phe", (9, 8, 1), dict(DT=1e-4, SOURCE=2), False),
     ("3phe", (1, 12, 9), dict(DT=1e-4), False),
     ("3phe", (3, 3, 3), dict(DT=1e-4), False),
+    ("3phe", (20, 18, 16), dict(DT=1e-4, CLOSURE="corey", S_WR_0=0.1, S_NR_0=0.05), False),
     ("3ph", (3, 3, 3), dict(CLOSURE="corey"), True),
     ("3ph", (1, 15, 1), dict(), False),
     ("3ph", (7, 3, 1), dict(SOURCE=2, CLOSURE="corey"), True),
@@ -166,9 +167,10 @@ def test_full_size_fused_equals_sequence():
     assert np.all(np.isfinite(pa["P_w"]))
 
 
+@pytest.mark.parametrize("closure", ["reference", "corey"])
 @pytest.mark.parametrize("dims", [(20, 18, 16), (9, 8, 1), (1, 12, 9), (300, 5, 4), (130, 3, 3)])
-def test_energy_passes_equal_the_sequence(dims):
-    keys = case("3phe", *dims, DT=1e-4, SOURCE=1)
+def test_energy_passes_equal_the_sequence(dims, closure):
+    keys = case("3phe", *dims, DT=1e-4, SOURCE=1, CLOSURE=closure)
     st = random_state(keys, seed=sum(dims) + 11)
     seq = capaz.Domain(keys, st)
     seq.tune(energy_kernel=0)
