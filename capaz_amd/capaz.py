@@ -57,6 +57,8 @@ EXPORTS = {
     "capaz_load_face": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "capaz_save_face": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "capaz_save_state": (ctypes.c_int, ["ctx", ctypes.c_char_p, ctypes.c_long]),
+    "capaz_write_plot_rows": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.c_long,
+                                             ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
     "capaz_load_state": (ctypes.c_int, ["ctx", ctypes.c_char_p, ctypes.POINTER(ctypes.c_long)]),
     "capaz_objective": (ctypes.c_int, ["ctx", ctypes.c_long, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "capaz_adjoint": (ctypes.c_int, ["ctx", ctypes.c_long, ctypes.c_void_p, ctypes.c_int,

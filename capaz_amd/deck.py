@@ -26,8 +26,8 @@ out).
 ``save_data_plots`` writes plots/S=%012.4f.dat as print_plots_top /
 print_plots / print_plot_row do for one domain (main.cpp:474-780): the same
 header lines, rows in (i, j, k) order with k fastest, y printed as
-(Ny - 1 - j) * hy and u_y negated, and the same printf formats (Python's
-%-formatting of doubles is C's).
+(Ny - 1 - j) * hy and u_y negated, and the same printf formats, the rows
+printed by the library's C loop (capaz_write_plot_rows).
 """
 from __future__ import annotations
 
@@ -232,25 +232,48 @@ def _layout(keys: dict):
             head = 'VARIABLES = "X","Y","Z","S_n","P_w","u_x", "u_y", "u_z", "porosity" \n'
             cols = lambda f, X, Y, Z: [X, Z, Y, f["S_n"], f["P_w"], f["ux_n"], f["uz_n"], -f["uy_n"], f["m"]]
             fmts = [e2] * 3 + [e3] * 6
-    return head, zone, " ".join(fmts) + "\n", cols
+    return head, zone, fmts, cols
 
 
-def plot_text(fields: dict, keys: dict, t: float) -> str:
-    """The text of one plot file for the fields (each (Nz, Ny, Nx))."""
+def plot_table(fields: dict, keys: dict, t: float):
+    """The three header lines, the per-column printf formats and the rows
+    (an array, one row per cell in print_plots' i, j, k order, k fastest) of
+    one plot file for the fields (each (Nz, Ny, Nx))."""
     nx, ny, nz = int(keys["NX"]), int(keys["NY"]), int(keys["NZ"])
-    head, zone, fmt, cols = _layout(keys)
-    # rows in the order of print_plots' i, j, k loops (k fastest)
+    head, zone, fmts, cols = _layout(keys)
     order = lambda a: np.asarray(a, dtype=np.float64).transpose(2, 1, 0).reshape(-1)
     f = {n: order(v) for n, v in fields.items()}
     i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
     X = i.reshape(-1) * float(keys["HX"])
     Y = (ny - 1 - j.reshape(-1)) * float(keys["HY"])
     Z = k.reshape(-1) * float(keys["HZ"])
-    table = np.stack(cols(f, X, Y, Z), axis=1)
-    # negating a zero velocity prints "-0.00000e+00", as (-1)*u does in C
-    out = ['TITLE =  "Filtration in time=%5.2f" \n' % t, head, zone]
-    out.extend(fmt % tuple(r) for r in table.tolist())
-    return "".join(out)
+    # negating a zero velocity gives -0.0, printed "-0.00000e+00" as (-1)*u in C
+    table = np.ascontiguousarray(np.stack(cols(f, X, Y, Z), axis=1))
+    return ['TITLE =  "Filtration in time=%5.2f" \n' % t, head, zone], fmts, table
+
+
+def plot_text(fields: dict, keys: dict, t: float) -> str:
+    """The text of one plot file, formatted in Python (small grids, tests)."""
+    lines, fmts, table = plot_table(fields, keys, t)
+    fmt = " ".join(fmts) + "\n"
+    return "".join(lines) + "".join(fmt % tuple(r) for r in table.tolist())
+
+
+def write_plot(path: str, fields: dict, keys: dict, t: float) -> None:
+    """Write one plot file: the header from here, the rows by the C library's
+    printf (capaz_write_plot_rows), as print_plot_row writes them."""
+    import ctypes
+
+    from .capaz import load
+
+    lines, fmts, table = plot_table(fields, keys, t)
+    with open(path, "wt") as fp:
+        fp.write("".join(lines))
+    fm = (ctypes.c_char_p * len(fmts))(*[f.encode() for f in fmts])
+    rc = load().capaz_write_plot_rows(os.fsencode(path), table.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                      table.shape[0], table.shape[1], fm)
+    if rc:
+        raise OSError(load().capaz_last_error(None).decode())
 
 
 PLOT_FIELDS = ("S_w", "S_n", "S_g", "P_w", "T", "m", "ux_w", "uy_w", "uz_w", "ux_n", "uy_n", "uz_n", "ux_g",
@@ -265,8 +288,7 @@ def save_data_plots(domain, keys: dict, t: float, workdir: str = ".") -> str:
     d = os.path.join(workdir, "plots")
     os.makedirs(d, exist_ok=True)
     path = os.path.join(d, plot_name(t))
-    with open(path, "wt") as fp:
-        fp.write(plot_text(fields, keys, t))
+    write_plot(path, fields, keys, t)
     return path
 
 

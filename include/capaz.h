@@ -214,6 +214,16 @@ int capaz_save_face(capaz_ctx* ctx, int field, int axis, int side, const double*
 int capaz_save_state(capaz_ctx* ctx, const char* path, long step);
 int capaz_load_state(capaz_ctx* ctx, const char* path, long* step);
 
+/* ---- Plot files ----------------------------------------------------------------
+ * Appends nrows rows of a row-major nrows x ncols table to the text file at
+ * path, each value printed with its column's printf format (one double
+ * conversion each, e.g. "%.5e"), separated by one space, rows ending in '\n':
+ * the row loop of print_plot_row (main.cpp:700-780) on host data, for
+ * capaz_amd.deck's Tecplot writer.  Needs no context; errors are read with
+ * capaz_last_error(NULL). */
+int capaz_write_plot_rows(const char* path, const double* table, long nrows, int ncols,
+                          const char* const* formats);
+
 /* ---- History matching: objective and discrete adjoint ----------------------
  * No counterpart in the reference.  The objective of nsteps fused steps from
  * the context's current state is

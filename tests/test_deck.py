@@ -187,6 +187,27 @@ def test_plot_layout_matches_print_plot_row():
     assert len(lines) == 3 + 12 and deck.plot_name(1.5) == "S=0000001.5000.dat"
 
 
+def test_c_row_writer_matches_python_formatting_and_checks_formats(tmp_path):
+    import ctypes
+
+    from capaz_amd import capaz
+
+    keys = case("3phe", 3, 4, 5)
+    rng = np.random.default_rng(5)
+    shape = (5, 4, 3)
+    f = {n: rng.standard_normal(shape) * 10.0 ** rng.integers(-8, 8, shape) for n in deck.PLOT_FIELDS}
+    f["uy_w"][0, 0, 0] = 0.0
+    path = str(tmp_path / "p.dat")
+    deck.write_plot(path, f, keys, 0.25)
+    assert open(path).read() == deck.plot_text(f, keys, 0.25)
+    L = capaz.load()
+    one = np.zeros(1)
+    for bad in (b"%d", b"%s", b"%.2e %e", b"x", b"%*e"):
+        fm = (ctypes.c_char_p * 1)(bad)
+        assert L.capaz_write_plot_rows(path.encode(), one.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 1, 1,
+                                       fm) == -1, bad
+
+
 @pytest.mark.parametrize("model,keys", DECKS, ids=IDS)
 def test_deck_on_the_oracle_writes_the_reference_programs_plots(model, keys, tmp_path):
     from oracle import oracle
