@@ -9,7 +9,10 @@ grids the oracle finishes in seconds.
     mu(T) and lambda_g(T) call exp/pow, the device library's, not glibc's)
     at 24 x 20 x 16, 300 steps: every
     field within 1e-9 of the oracle relative to the field's magnitude, and
-    P_w, S_w, S_n element by element -- the full-run bar.
+    P_w, S_w, S_n element by element -- the full-run bar;
+  * the ensemble and inversion workloads at their specified sizes, through
+    properties that hold at any size: batched = single-domain bits per
+    realization; gradient . v = tangent derivative along v to rounding.
 """
 import numpy as np
 import pytest
@@ -51,3 +54,59 @@ def test_energy_whole_run_within_1e9(closure):
         assert norm_rel(got[f], v) <= 1e-9, (f, norm_rel(got[f], v))
         if f in STATE:
             assert max_rel(got[f], v) <= 1e-9, (f, max_rel(got[f], v))
+
+
+def test_ensemble_full_size_batched_equals_single_domain_runs():
+    """The ensemble workload as specified (64 two-phase realizations of
+    edges 32-96, 500 steps): one batched run gives every realization the bits
+    of its own single-domain run."""
+    keys, reals = workloads.ensemble(count=64)
+    b = capaz.Batch(keys, reals)
+    b.run(500)
+    for r, m in enumerate(reals):
+        nx, ny, nz = m["dims"]
+        d = capaz.Domain(dict(keys, NX=nx, NY=ny, NZ=nz), {f: v for f, v in m.items() if f != "dims"})
+        d.run(500)
+        for f in ("P_w", "S_n"):
+            assert bit_equal(b.domains[r].field(f), d.field(f)), (r, m["dims"], f)
+        d.close()
+    b.close()
+
+
+def test_inversion_full_size_gradient_matches_the_tangent():
+    """The inversion workload as specified (96^3, 400 steps, 16 observation
+    columns every 50 steps): along random directions in (ln K, m) the
+    gradient's inner product equals the tangent-linear derivative to
+    rounding, for the default segment and for sqrt(N) checkpoints alike."""
+    keys, st, lnK_true, lnK0, ob = workloads.history_matching()
+    t = capaz.Domain(keys, dict(st, K=np.exp(lnK_true)))
+    seen_w, seen_n, done = [], [], 0
+    for s in ob["steps"]:
+        t.run(int(s) - done, first_step=done + 1)
+        done = int(s)
+        seen_w.append(t.field("S_w").ravel()[ob["cells"]])
+        seen_n.append(t.field("S_n").ravel()[ob["cells"]])
+    t.close()
+    obs = dict(ob, S_w=np.array(seen_w), S_n=np.array(seen_n))
+    d = capaz.Domain(keys, dict(st, K=np.exp(lnK0)))
+
+    def reset():
+        for f, v in st.items():
+            d.set_field(f, v)
+        d.set_field("K", np.exp(lnK0))
+
+    reset()
+    J, gK, gm = d.adjoint(400, obs)
+    reset()
+    J2, gK2, gm2 = d.adjoint(400, obs, checkpoint_every=-1)
+    assert J == J2 and bit_equal(gK, gK2) and bit_equal(gm, gm2)
+    rng = np.random.default_rng(7)
+    for _ in range(2):
+        vK = rng.standard_normal(lnK0.shape)
+        vm = 0.02 * rng.standard_normal(lnK0.shape)
+        reset()
+        Jt, dJ = d.tangent(400, obs, vK, vm)
+        ad = float(np.sum(gK * vK) + np.sum(gm * vm))
+        assert Jt == J
+        assert abs(dJ - ad) <= 1e-12 * abs(dJ), (dJ, ad)
+    d.close()
