@@ -323,11 +323,14 @@ CAPAZ_HD void body_boundary(const DevParams& p, const DevFields& F, int64_t l) {
 //                      Xi, ro, H, E in place), and the cell's lambda_eff into
 //                      a work array: E's conduction reads it at 7 points per
 //                      cell instead of evaluating it (a pow) 9 times;
-//   B (all cells)      u;   C (interior) roS;   E (interior) E_new;
-//   N (interior)       the Newton update;   D (boundary faces) the boundary.
-// The stencil passes stay apart: fused, their ~30 arrays read at 7 points
-// overflow L2 and run slower than apart, and the Newton update holds ~200
-// registers, which would cut the stencils' occupancy.
+//   U (boundary cells) u;
+//   C (interior)       roS and u on the fused kernel's z tiles, from the
+//                      coefficients A stored (k_energy_roS_z, capaz_fused.h);
+//   E (interior)       E_new;   N (interior) the Newton update;
+//   D (boundary cells) the boundary.
+// E stays apart from C: fused per cell, their ~30 arrays read at 7 points
+// ran slower than apart; the Newton update holds ~200 registers, which would
+// cut the stencils' occupancy.
 
 CAPAZ_HD void body_energy_a(const DevParams& p, const DevFields& F, double* lam, int64_t l) {
     body_S(p, F, l);
