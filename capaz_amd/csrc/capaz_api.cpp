@@ -319,6 +319,14 @@ int capaz_roS(capaz_ctx* c, double t) {
     int rc;
     if ((rc = check_ctx(c))) return rc;
     if ((rc = materialize(c))) return rc;
+    if (c->user.model == 3) {
+        // three phases: on the z tiles unless some interior cell has m == 0
+        if ((rc = scan_m(c))) return rc;
+        if (!c->m_has_zero) {
+            CAPAZ_HIP(c, capaz::launch_roS_stored(c->p, fields_of(c), c->d_dtm, c->stream));
+            return CAPAZ_OK;
+        }
+    }
     return run_entry(c, capaz::Entry::RoS, fields_of(c));
 }
 

@@ -325,7 +325,7 @@ CAPAZ_HD void body_boundary(const DevParams& p, const DevFields& F, int64_t l) {
 //                      cell instead of evaluating it (a pow) 9 times;
 //   U (boundary cells) u;
 //   C (interior)       roS and u on the fused kernel's z tiles, from the
-//                      coefficients A stored (k_energy_roS_z, capaz_fused.h);
+//                      coefficients A stored (k_roS_z, capaz_fused.h);
 //   E (interior)       E_new;   N (interior) the Newton update;
 //   D (boundary cells) the boundary.
 // E stays apart from C: fused per cell, their ~30 arrays read at 7 points

@@ -389,14 +389,16 @@ struct FusedStoreRecord {
     }
 };
 
-// The energy step's roS pass (assign_roS_nr, cpu.cpp:619-731) on the z tiles
-// from the stored coefficients (CLOSURE 3): roS_old and roS of the cell, and
-// its velocities (assign_u) from the same neighbours.
-struct EnergyRoSStore {
-    static constexpr bool kWantsU = true;
+// assign_roS_nr (cpu.cpp:619-731) on the z tiles from the stored
+// coefficients (CLOSURE 3, three phases): roS_old and roS of the cell and,
+// with U, its velocities (assign_u) from the same neighbours -- the roS entry
+// point, and the energy step's roS + u pass.
+template <bool U>
+struct RoSStore {
+    static constexpr bool kWantsU = U;
     const double* dtm;
     double *roS[3], *roS_old[3];
-    double* u[3][3];  // [phase][axis]: ux_w, uy_w, uz_w, ux_n, ...
+    double* u[3][3];  // [phase][axis]: ux_w, uy_w, uz_w, ux_n, ... (U only)
     __device__ __forceinline__ void store_u(int64_t l, const double (&v)[3][3]) const {
 #pragma unroll
         for (int a = 0; a < 3; ++a)
@@ -416,8 +418,8 @@ struct EnergyRoSStore {
     }
 };
 
-template <int TX, int TY, bool CARRY>
-__global__ void __launch_bounds__(TX * TY) k_energy_roS_z(DevParams p, StepIO io, int kchunk, EnergyRoSStore st) {
+template <int TX, int TY, bool CARRY, bool U>
+__global__ void __launch_bounds__(TX * TY) k_roS_z(DevParams p, StepIO io, int kchunk, RoSStore<U> st) {
     __shared__ FusedLds<9, TX, TY> lds;
     fused_z_tile<3, 3, TX, TY, CARRY>(p, io, kchunk, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, lds, st);
 }

@@ -44,19 +44,14 @@ __global__ void __launch_bounds__(kBlock) k_newton(DevParams p, DevFields F) {
     int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (l < p.n) body_newton(p, F, l);
 }
-__global__ void __launch_bounds__(kBlock) k_boundary(DevParams p, DevFields F) {
-    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (l < p.n) body_boundary(p, F, l);
-}
 __global__ void __launch_bounds__(kBlock) k_fused_step(DevParams p, StepIO io) {
     int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (l < p.n) body_fused_step(p, io, l);
 }
 
-// Energy step passes of their own: A, E from the stored lambda_eff, u of the
-// boundary cells and D over the boundary cells (face_cell's numbering); roS
-// and the interior u run on the z tiles (capaz_fused.h), Newton is the entry
-// point.
+// Energy step passes of their own: A, E from the stored lambda_eff and u of
+// the boundary cells (face_cell's numbering); roS and the interior u run on
+// the z tiles (capaz_fused.h), Newton and the boundary are the entry points.
 __global__ void __launch_bounds__(kBlock) k_energy_a(DevParams p, DevFields F, double* lam) {
     int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (l < p.n) body_energy_a(p, F, lam, l);
@@ -70,7 +65,9 @@ __global__ void __launch_bounds__(kBlock) k_energy_u_faces(DevParams p, DevField
     if (face_cell(p, (int64_t)blockIdx.x * kBlock + threadIdx.x, i, j, k))
         body_u(p, F, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz);
 }
-__global__ void __launch_bounds__(kBlock) k_energy_d(DevParams p, DevFields F) {
+// boundary_conditions over the boundary cells only (the entry point and the
+// energy step's last pass)
+__global__ void __launch_bounds__(kBlock) k_boundary_faces(DevParams p, DevFields F) {
     int i, j, k;
     if (face_cell(p, (int64_t)blockIdx.x * kBlock + threadIdx.x, i, j, k))
         body_boundary(p, F, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz);
@@ -138,7 +135,9 @@ hipError_t launch_entry(Entry e, const DevParams& p, const DevFields& F, hipStre
         case Entry::HE: hipLaunchKernelGGL(k_HE, g, b, 0, s, p, F); break;
         case Entry::E: hipLaunchKernelGGL(k_E, g, b, 0, s, p, F); break;
         case Entry::Newton: hipLaunchKernelGGL(k_newton, g, b, 0, s, p, F); break;
-        case Entry::Boundary: hipLaunchKernelGGL(k_boundary, g, b, 0, s, p, F); break;
+        case Entry::Boundary:
+            hipLaunchKernelGGL(k_boundary_faces, grid_for(fused_face_cells(p)), b, 0, s, p, F);
+            break;
     }
     return hipGetLastError();
 }
@@ -247,6 +246,43 @@ hipError_t launch_fused_record(const DevParams& p, const StepIO& io, int kchunk,
     return hipGetLastError();
 }
 
+// roS (and, with U, u) of the interior cells on the z tiles, from the P, Xi
+// and ro arrays of this step (three phases).
+template <bool U>
+hipError_t launch_roS_tiles(const DevParams& p, const DevFields& F, const double* dtm, hipStream_t s) {
+    StepIO io{};
+    io.Pw = F.f[F_P_w];
+    io.Sw = F.f[F_S_w];
+    io.Sn = F.f[F_S_n];
+    io.Pn = F.f[F_P_n];
+    io.Pg = F.f[F_P_g];
+    io.m = F.f[F_m];
+    io.K = F.f[F_K];
+    io.dtm = dtm;
+    const int xi[3] = {F_Xi_w, F_Xi_n, F_Xi_g}, ro[3] = {F_ro_w, F_ro_n, F_ro_g};
+    const int rs[3] = {F_roS_w, F_roS_n, F_roS_g}, ro_old[3] = {F_roS_w_old, F_roS_n_old, F_roS_g_old};
+    const int uf[3][3] = {{F_ux_w, F_uy_w, F_uz_w}, {F_ux_n, F_uy_n, F_uz_n}, {F_ux_g, F_uy_g, F_uz_g}};
+    RoSStore<U> st{};
+    st.dtm = dtm;
+    for (int a = 0; a < 3; ++a) {
+        io.Xi[a] = F.f[xi[a]];
+        io.ro[a] = F.f[ro[a]];
+        st.roS[a] = F.f[rs[a]];
+        st.roS_old[a] = F.f[ro_old[a]];
+        for (int x = 0; x < 3; ++x) st.u[a][x] = U ? F.f[uf[a][x]] : nullptr;
+    }
+    const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
+    const int kchunk = fused_kchunk(p, 0, 8, resident_groups<k_roS_z<64, 8, true, U>>(512));
+    dim3 grid((nx + 63) / 64, (ny + 7) / 8, (nz + kchunk - 1) / kchunk);
+    hipLaunchKernelGGL((k_roS_z<64, 8, true, U>), grid, dim3(64, 8), 0, s, p, io, kchunk, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_roS_stored(const DevParams& p, const DevFields& F, const double* dtm, hipStream_t s) {
+    if (p.model != 3) return hipErrorInvalidValue;
+    return launch_roS_tiles<false>(p, F, dtm, s);
+}
+
 hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* lam, const double* dtm, hipStream_t s,
                               const KernelEvents* ev) {
     hipError_t e;
@@ -255,37 +291,10 @@ hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* la
     hipLaunchKernelGGL(k_energy_a, g, b, 0, s, p, F, lam);
     // u of the boundary cells here, of the interior cells in the roS tiles
     hipLaunchKernelGGL(k_energy_u_faces, grid_for(fused_face_cells(p)), b, 0, s, p, F);
-    {
-        // roS on the z tiles from the coefficients pass A stored
-        StepIO io{};
-        io.Pw = F.f[F_P_w];
-        io.Sw = F.f[F_S_w];
-        io.Sn = F.f[F_S_n];
-        io.Pn = F.f[F_P_n];
-        io.Pg = F.f[F_P_g];
-        io.m = F.f[F_m];
-        io.K = F.f[F_K];
-        io.dtm = dtm;
-        const int xi[3] = {F_Xi_w, F_Xi_n, F_Xi_g}, ro[3] = {F_ro_w, F_ro_n, F_ro_g};
-        const int rs[3] = {F_roS_w, F_roS_n, F_roS_g}, ro_old[3] = {F_roS_w_old, F_roS_n_old, F_roS_g_old};
-        const int uf[3][3] = {{F_ux_w, F_uy_w, F_uz_w}, {F_ux_n, F_uy_n, F_uz_n}, {F_ux_g, F_uy_g, F_uz_g}};
-        EnergyRoSStore st{};
-        st.dtm = dtm;
-        for (int a = 0; a < 3; ++a) {
-            io.Xi[a] = F.f[xi[a]];
-            io.ro[a] = F.f[ro[a]];
-            st.roS[a] = F.f[rs[a]];
-            st.roS_old[a] = F.f[ro_old[a]];
-            for (int x = 0; x < 3; ++x) st.u[a][x] = F.f[uf[a][x]];
-        }
-        const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
-        const int kchunk = fused_kchunk(p, 0, 8, resident_groups<k_energy_roS_z<64, 8, true>>(512));
-        dim3 grid((nx + 63) / 64, (ny + 7) / 8, (nz + kchunk - 1) / kchunk);
-        hipLaunchKernelGGL((k_energy_roS_z<64, 8, true>), grid, dim3(64, 8), 0, s, p, io, kchunk, st);
-    }
+    if ((e = launch_roS_tiles<true>(p, F, dtm, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_energy_e, g, b, 0, s, p, F, lam);
     hipLaunchKernelGGL(k_newton, g, b, 0, s, p, F);
-    hipLaunchKernelGGL(k_energy_d, grid_for(fused_face_cells(p)), b, 0, s, p, F);
+    hipLaunchKernelGGL(k_boundary_faces, grid_for(fused_face_cells(p)), b, 0, s, p, F);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return ev ? hipEventRecord(ev->end, s) : hipSuccess;
 }

@@ -40,6 +40,10 @@ hipError_t launch_entry(Entry e, const DevParams& p, const DevFields& F, hipStre
 hipError_t launch_fused_step(const DevParams& p, const StepIO& io, hipStream_t s);
 hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s,
                         const KernelEvents* ev = nullptr);
+// The roS entry point of the three-phase model on the fused kernel's z tiles,
+// from the P, Xi and ro arrays P_Xi_ro wrote (no cell may have m == 0: the
+// per-cell entry point keeps their stale roS); dtm holds dt / m per cell.
+hipError_t launch_roS_stored(const DevParams& p, const DevFields& F, const double* dtm, hipStream_t s);
 // One step of the energy model in six passes, in place on F; lam is a work
 // array of Nx*Ny*Nz doubles, dtm holds dt / m per cell; ev brackets the whole
 // step.
