@@ -306,6 +306,20 @@ __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& i
     }
 }
 
+// Tile of workgroup (blockIdx): workgroups go to the 8 XCDs round robin, so
+// when the grid divides by 8 each XCD takes one contiguous run of tiles (x
+// fastest, then y, then z) and the halo rows its tiles share stay in its L2.
+__device__ __forceinline__ void xcd_tile(int& bx, int& by, int& bz) {
+    const uint32_t gx = gridDim.x, gy = gridDim.y;
+    const uint32_t nb = gx * gy * gridDim.z;
+    uint32_t id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    if (nb % 8u == 0u) id = (id % 8u) * (nb / 8u) + id / 8u;
+    bx = (int)(id % gx);
+    const uint32_t r = id / gx;
+    by = (int)(r % gy);
+    bz = (int)(r / gy);
+}
+
 // The boundary cells whose inward neighbour (set_boundary_basic_coordinate,
 // cpu.cpp:107-141) is the interior cell (i, j, k), from its new state: the
 // cell's offsets along each axis are 0 and -1 next to the low face, +1 next to
@@ -421,14 +435,18 @@ struct RoSStore {
 template <int TX, int TY, bool CARRY, bool U>
 __global__ void __launch_bounds__(TX * TY) k_roS_z(DevParams p, StepIO io, int kchunk, RoSStore<U> st) {
     __shared__ FusedLds<9, TX, TY> lds;
-    fused_z_tile<3, 3, TX, TY, CARRY>(p, io, kchunk, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, lds, st);
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    fused_z_tile<3, 3, TX, TY, CARRY>(p, io, kchunk, bx, by, bz, lds, st);
 }
 
 template <int MODEL, int CLOSURE, int TX, int TY, bool CARRY>
 __global__ void __launch_bounds__(TX * TY) k_fused_z_record(DevParams p, StepIO io, int kchunk, double* div,
                                                            int64_t stride) {
     __shared__ FusedLds<3 * MODEL, TX, TY> lds;
-    fused_z_tile<MODEL, CLOSURE, TX, TY, CARRY>(p, io, kchunk, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, lds,
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    fused_z_tile<MODEL, CLOSURE, TX, TY, CARRY>(p, io, kchunk, bx, by, bz, lds,
                                                 FusedStoreRecord<MODEL>{FusedStore<MODEL>{io}, div, stride});
 }
 
@@ -440,7 +458,9 @@ constexpr bool kFusedFaces = MODEL == 2;
 template <int MODEL, int CLOSURE, int TX, int TY, bool CARRY>
 __global__ void __launch_bounds__(TX * TY) k_fused_z(DevParams p, StepIO io, int kchunk) {
     __shared__ FusedLds<3 * MODEL, TX, TY> lds;
-    fused_z_tile<MODEL, CLOSURE, TX, TY, CARRY>(p, io, kchunk, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, lds,
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    fused_z_tile<MODEL, CLOSURE, TX, TY, CARRY>(p, io, kchunk, bx, by, bz, lds,
                                                 FusedStore<MODEL, kFusedFaces<MODEL>>{io});
 }
 
