@@ -25,7 +25,11 @@ __global__ void __launch_bounds__(TX * TY)
               int kchunk, int* counter) {
     __shared__ FusedLds<3 * MODEL, TX, TY> lds;
     __shared__ int claimed;
+    // one workgroup per tile: each XCD (workgroups go to the 8 XCDs round
+    // robin) takes a contiguous run of the tile list, so a realization's
+    // neighbouring tiles share its L2
     int t = (int)blockIdx.x;
+    if (!counter && gridDim.x % 8u == 0u) t = (int)((blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u);
     for (;;) {
         if (counter) {
             if (threadIdx.x == 0 && threadIdx.y == 0) claimed = atomicAdd(counter, 1);
@@ -59,7 +63,9 @@ template <int MODEL, int CLOSURE, int TX, int TY>
 hipError_t step_shape(const BatchShape& sh, const BatchDomain* doms, const int* tile_owner, int ntiles,
                       const int* face_owner, int nface_blocks, int parity, int* counter, int grid, hipStream_t s) {
     if (ntiles > 0) {
-        const int g = sh.persistent ? grid : ntiles;
+        // (the one-per-tile grid is rounded up to a multiple of the 8 XCDs;
+        // the extra workgroups find no tile and exit)
+        const int g = sh.persistent ? grid : (ntiles + 7) / 8 * 8;
         hipLaunchKernelGGL((k_batch_z<MODEL, CLOSURE, TX, TY>), dim3(g), dim3(TX, TY), 0, s, doms, tile_owner, ntiles,
                            parity, sh.kchunk, sh.persistent ? counter : nullptr);
         hipError_t e = hipGetLastError();
