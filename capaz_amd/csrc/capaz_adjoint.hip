@@ -35,8 +35,12 @@ __global__ void __launch_bounds__(256) k_adj_newton_rec(DevParams p, AdjIO a, co
 // Every cell, in memory order.  Four waves per SIMD (128 VGPRs; the
 // compiler's own choice of 130 allows three) hide more of the neighbours'
 // load latency, at the cost of a 12-byte spill.
+// The grid is a multiple of the 8 XCDs; each XCD takes a contiguous run of
+// cells (workgroups go to the XCDs round robin), so the z neighbours a cell
+// gathers from were read by the same XCD and hit its L2.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_adj_coef(DevParams p, AdjIO a) {
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t b = (blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u;
+    const uint32_t t = b * 256u + threadIdx.x;
     if (t >= (uint32_t)p.n) return;
     int i, j, k;
     box_cell(t, (uint32_t)p.Nx, (uint32_t)p.Ny, 0, 0, 0, i, j, k);
@@ -242,7 +246,7 @@ hipError_t launch_adjoint_step_tiled(const DevParams& p, const AdjIO& a, hipStre
 }
 
 hipError_t launch_adjoint_coef(const DevParams& p, const AdjIO& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_coef, dim3((unsigned)((p.n + 255) / 256)), dim3(256), 0, s, p, a);
+    hipLaunchKernelGGL(k_adj_coef, dim3((unsigned)((p.n + 255) / 256 + 7) / 8 * 8), dim3(256), 0, s, p, a);
     return hipGetLastError();
 }
 
