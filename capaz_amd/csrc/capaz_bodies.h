@@ -183,15 +183,11 @@ CAPAZ_HD double lambda_eff_at(const DevFields& F, int64_t l) {
     return lambda_eff(F.f[F_m][l], F.f[F_S_w][l], F.f[F_S_n][l], F.f[F_S_g][l], F.f[F_T][l]);
 }
 
-// lambda_eff of a cell: computed from the fields, or read from an array that
-// holds the same values computed earlier in the step.
+// lambda_eff of a cell, computed from the fields (the energy step's tiled E
+// pass reads the same values from an array instead).
 struct LamOfFields {
     const DevFields& F;
     CAPAZ_HD double operator()(int64_t l) const { return lambda_eff_at(F, l); }
-};
-struct LamOfArray {
-    const double* lam;
-    CAPAZ_HD double operator()(int64_t l) const { return lam[l]; }
 };
 
 template <class Lam>
@@ -326,7 +322,9 @@ CAPAZ_HD void body_boundary(const DevParams& p, const DevFields& F, int64_t l) {
 //   U (boundary cells) u;
 //   C (interior)       roS and u on the fused kernel's z tiles, from the
 //                      coefficients A stored (k_roS_z, capaz_fused.h);
-//   E (interior)       E_new;   N (interior) the Newton update;
+//   E (interior)       E_new on z tiles, from lambda_eff * T and the advected
+//                      products staged in LDS (k_energy_e_z, capaz_fused.h);
+//   N (interior)       the Newton update;
 //   D (boundary cells) the boundary.
 // E stays apart from C: fused per cell, their ~30 arrays read at 7 points
 // ran slower than apart; the Newton update holds ~200 registers, which would

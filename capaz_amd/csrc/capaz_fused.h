@@ -440,6 +440,116 @@ __global__ void __launch_bounds__(TX * TY) k_roS_z(DevParams p, StepIO io, int k
     fused_z_tile<3, 3, TX, TY, CARRY>(p, io, kchunk, bx, by, bz, lds, st);
 }
 
+// The energy step's E pass (assign_E_new, energy.cpp:235-340, as body_E_with)
+// on z-marching TX x TY tiles: per plane, each cell's lambda_eff * T and its
+// advected products (ro_a * H_a) * ux_a and (ro_a * H_a) * uy_a are staged in
+// LDS with a one-cell halo ring; the z neighbours' lambda_eff * T and
+// (ro_a * H_a) * uy_a (the z term advects with uy, energy.cpp:300-305) are
+// computed per column.  Each term is the product the per-cell body forms, so
+// E_new has its bits.
+struct EnergyEIO {
+    const double *lam, *T, *E;
+    const double *ro[3], *H[3], *ux[3], *uy[3];
+    double* E_new;
+};
+
+// v[0] = lambda_eff * T, v[1 + a] = (ro_a H_a) ux_a, v[4 + a] = (ro_a H_a) uy_a
+__device__ __forceinline__ void energy_e_vals(const EnergyEIO& a, int64_t l, double* v) {
+    v[0] = a.lam[l] * a.T[l];
+#pragma unroll
+    for (int ph = 0; ph < 3; ++ph) {
+        const double rh = a.ro[ph][l] * a.H[ph][l];
+        v[1 + ph] = rh * a.ux[ph][l];
+        v[4 + ph] = rh * a.uy[ph][l];
+    }
+}
+
+// lambda_eff * T and the uy products of a z neighbour: w[0], w[1 + a]
+__device__ __forceinline__ void energy_e_zvals(const EnergyEIO& a, int64_t l, double* w) {
+    w[0] = a.lam[l] * a.T[l];
+#pragma unroll
+    for (int ph = 0; ph < 3; ++ph) w[1 + ph] = a.ro[ph][l] * a.H[ph][l] * a.uy[ph][l];
+}
+
+// advection's (((X_w+ - X_w-) + X_n+) - X_n-) + X_g+) - X_g- (the body's order)
+__device__ __forceinline__ double energy_adv(double wp, double wm, double np_, double nm, double gp, double gm) {
+    return wp - wm + np_ - nm + gp - gm;
+}
+
+template <int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) k_energy_e_z(DevParams p, EnergyEIO a, int kchunk) {
+    static_assert(TY >= 3 && TX >= 2 * TY, "halo ring needs TY >= 3 and TX >= 2*TY");
+    __shared__ double lds[7][TY + 2][TX + 2];
+    int bx, by, bz;
+    xcd_tile(bx, by, bz);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const bool hasx = p.Nx >= 3, hasz = p.Nz >= 3;
+    const int ilo = hasx ? 1 : 0, ihi = hasx ? p.Nx - 2 : 0;
+    const int klo = hasz ? 1 : 0, khi = hasz ? p.Nz - 2 : 0;
+    const int i0 = ilo + TX * bx, j0 = 1 + TY * by;
+    const int k0 = klo + kchunk * bz;
+    const int k1 = min(khi, k0 + kchunk - 1);
+    const int i = i0 + tx, j = j0 + ty;
+    const bool own_in_grid = i < p.Nx && j < p.Ny;
+    const bool active = i <= ihi && j <= p.Ny - 2;
+    // halo cell of this thread, as in fused_z_tile
+    int hi = -1, hj = -1, hx = 0, hy = 0;
+    if (ty == 0) { hi = i; hj = j0 - 1; hx = tx + 1; hy = 0; }
+    else if (ty == 1) { hi = i; hj = j0 + TY; hx = tx + 1; hy = TY + 1; }
+    else if (ty == 2 && tx < TY) { hi = i0 - 1; hj = j0 + tx; hx = 0; hy = tx + 1; }
+    else if (ty == 2 && tx >= TY && tx < 2 * TY) { hi = i0 + TX; hj = j0 + tx - TY; hx = TX + 1; hy = tx - TY + 1; }
+    const bool halo_in_grid = hi >= 0 && hi < p.Nx && hj >= 0 && hj < p.Ny && (hasx || hi == i0 + tx);
+    const int64_t col = (int64_t)i + (int64_t)j * p.sy;
+    const int64_t hcol = (int64_t)hi + (int64_t)hj * p.sy;
+
+    double zm[4] = {}, zp[4] = {};
+    if (own_in_grid && hasz) energy_e_zvals(a, col + (int64_t)(k0 - 1) * p.sz, zm);
+    for (int k = k0; k <= k1; ++k) {
+        const int64_t l = col + (int64_t)k * p.sz;
+        double v[7] = {};
+        if (own_in_grid) {
+            energy_e_vals(a, l, v);
+#pragma unroll
+            for (int q = 0; q < 7; ++q) lds[q][ty + 1][tx + 1] = v[q];
+        }
+        if (hi >= 0 && halo_in_grid) {
+            double h[7];
+            energy_e_vals(a, hcol + (int64_t)k * p.sz, h);
+#pragma unroll
+            for (int q = 0; q < 7; ++q) lds[q][hy][hx] = h[q];
+        }
+        __syncthreads();
+        if (own_in_grid && hasz) energy_e_zvals(a, l + p.sz, zp);
+        if (active) {
+            const double lc = a.lam[l], Tc = a.T[l];
+            double T_flow = 0;
+            if (p.Nx > 2)
+                T_flow += exact_div(lds[0][ty + 1][tx + 2] - 2 * lc * Tc + lds[0][ty + 1][tx], p.hx2, p.r_hx2);
+            if (p.Ny > 2)
+                T_flow += exact_div(lds[0][ty + 2][tx + 1] - 2 * lc * Tc + lds[0][ty][tx + 1], p.hy2, p.r_hy2);
+            if (p.Nz > 2) T_flow = exact_div(zp[0] - 2 * lc * Tc + zm[0], p.hz2, p.r_hz2);  // '=' as energy.cpp:257
+            double E_flow = 0;
+            if (p.Nx > 2)
+                E_flow += exact_div(energy_adv(lds[1][ty + 1][tx + 2], lds[1][ty + 1][tx], lds[2][ty + 1][tx + 2],
+                                               lds[2][ty + 1][tx], lds[3][ty + 1][tx + 2], lds[3][ty + 1][tx]),
+                                    p.two_hx, p.r_two_hx);
+            if (p.Ny > 2)
+                E_flow += exact_div(energy_adv(lds[4][ty + 2][tx + 1], lds[4][ty][tx + 1], lds[5][ty + 2][tx + 1],
+                                               lds[5][ty][tx + 1], lds[6][ty + 2][tx + 1], lds[6][ty][tx + 1]),
+                                    p.two_hy, p.r_two_hy);
+            if (p.Nz > 2)
+                E_flow += exact_div(energy_adv(zp[1], zm[1], zp[2], zm[2], zp[3], zm[3]), p.two_hz, p.r_two_hz);
+            double Q_hw = 0, Q_hr = 0;
+            a.E_new[l] = a.E[l] + p.dt * (T_flow + Q_hw + Q_hr - E_flow);
+        }
+        __syncthreads();
+        // roll the z window: the centre's lambda_eff * T and uy products
+        zm[0] = v[0];
+#pragma unroll
+        for (int ph = 0; ph < 3; ++ph) zm[1 + ph] = v[4 + ph];
+    }
+}
+
 template <int MODEL, int CLOSURE, int TX, int TY, bool CARRY>
 __global__ void __launch_bounds__(TX * TY) k_fused_z_record(DevParams p, StepIO io, int kchunk, double* div,
                                                            int64_t stride) {

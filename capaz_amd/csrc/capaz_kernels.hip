@@ -49,16 +49,12 @@ __global__ void __launch_bounds__(kBlock) k_fused_step(DevParams p, StepIO io) {
     if (l < p.n) body_fused_step(p, io, l);
 }
 
-// Energy step passes of their own: A, E from the stored lambda_eff and u of
-// the boundary cells (face_cell's numbering); roS and the interior u run on
-// the z tiles (capaz_fused.h), Newton and the boundary are the entry points.
+// Energy step passes of their own: A and u of the boundary cells (face_cell's
+// numbering); roS with the interior u, and E, run on z tiles (capaz_fused.h);
+// Newton and the boundary are the entry points.
 __global__ void __launch_bounds__(kBlock) k_energy_a(DevParams p, DevFields F, double* lam) {
     int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     if (l < p.n) body_energy_a(p, F, lam, l);
-}
-__global__ void __launch_bounds__(kBlock) k_energy_e(DevParams p, DevFields F, const double* lam) {
-    int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (l < p.n) body_E_with(p, F, LamOfArray{lam}, l);
 }
 __global__ void __launch_bounds__(kBlock) k_energy_u_faces(DevParams p, DevFields F) {
     int i, j, k;
@@ -292,7 +288,26 @@ hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* la
     // u of the boundary cells here, of the interior cells in the roS tiles
     hipLaunchKernelGGL(k_energy_u_faces, grid_for(fused_face_cells(p)), b, 0, s, p, F);
     if ((e = launch_roS_tiles<true>(p, F, dtm, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_energy_e, g, b, 0, s, p, F, lam);
+    {
+        // E_new on the z tiles
+        EnergyEIO ea{};
+        ea.lam = lam;
+        ea.T = F.f[F_T];
+        ea.E = F.f[F_E];
+        ea.E_new = F.f[F_E_new];
+        const int ro[3] = {F_ro_w, F_ro_n, F_ro_g}, H[3] = {F_H_w, F_H_n, F_H_g};
+        const int ux[3] = {F_ux_w, F_ux_n, F_ux_g}, uy[3] = {F_uy_w, F_uy_n, F_uy_g};
+        for (int a = 0; a < 3; ++a) {
+            ea.ro[a] = F.f[ro[a]];
+            ea.H[a] = F.f[H[a]];
+            ea.ux[a] = F.f[ux[a]];
+            ea.uy[a] = F.f[uy[a]];
+        }
+        const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
+        const int kc = fused_kchunk(p, 0, 8, resident_groups<k_energy_e_z<64, 8>>(512));
+        dim3 ge((nx + 63) / 64, (ny + 7) / 8, (nz + kc - 1) / kc);
+        hipLaunchKernelGGL((k_energy_e_z<64, 8>), ge, dim3(64, 8), 0, s, p, ea, kc);
+    }
     hipLaunchKernelGGL(k_newton, g, b, 0, s, p, F);
     hipLaunchKernelGGL(k_boundary_faces, grid_for(fused_face_cells(p)), b, 0, s, p, F);
     if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -19,8 +19,9 @@ enum class Entry { S, P_Xi_ro, U, RoS, HE, E, Newton, Boundary };
 // cell; 1 = both passes on the z-marching LDS tiles; 0 = both per cell,
 // recomputing the neighbours' coefficients.
 // energy_kernel: 1 = the energy model's step in six passes (S, P_Xi_ro, HE
-// and lambda_eff per cell; u; roS; E from the stored lambda_eff; Newton;
-// boundary faces), 0 = the eight entry-point kernels.
+// and lambda_eff per cell; boundary u; roS and interior u on z tiles; E on z
+// tiles from the stored lambda_eff; Newton; boundary faces), 0 = the eight
+// entry-point kernels.
 struct Tuning {
     int fused_kernel = 1;
     int tile_y = 8;
