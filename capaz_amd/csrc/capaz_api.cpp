@@ -310,7 +310,16 @@ CAPAZ_ENTRY(capaz_S, S, true)
 CAPAZ_ENTRY(capaz_P_Xi_ro, P_Xi_ro, true)
 CAPAZ_ENTRY(capaz_u, U, true)
 CAPAZ_ENTRY(capaz_energy_HE, HE, c->user.energy)
-CAPAZ_ENTRY(capaz_energy_E, E, c->user.energy)
+
+int capaz_energy_E(capaz_ctx* c) {
+    int rc;
+    if ((rc = check_ctx(c))) return rc;
+    if (!c->user.energy) return fail(c, CAPAZ_ERR_STATE, "capaz_energy_E: not in this model");
+    if ((rc = materialize(c))) return rc;
+    // on the z tiles, lambda_eff evaluated once per staged cell
+    CAPAZ_HIP(c, capaz::launch_energy_e_tiles(c->p, fields_of(c), nullptr, c->stream));
+    return CAPAZ_OK;
+}
 CAPAZ_ENTRY(capaz_newton, Newton, true)
 CAPAZ_ENTRY(capaz_boundary, Boundary, true)
 

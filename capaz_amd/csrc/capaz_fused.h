@@ -447,15 +447,22 @@ __global__ void __launch_bounds__(TX * TY) k_roS_z(DevParams p, StepIO io, int k
 // (ro_a * H_a) * uy_a (the z term advects with uy, energy.cpp:300-305) are
 // computed per column.  Each term is the product the per-cell body forms, so
 // E_new has its bits.
+// lam: lambda_eff per cell as the step's first pass stored it, or NULL (the
+// entry point) to evaluate it from m, S_w, S_n, S_g and T.
 struct EnergyEIO {
     const double *lam, *T, *E;
+    const double *m, *Sw, *Sn, *Sg;
     const double *ro[3], *H[3], *ux[3], *uy[3];
     double* E_new;
 };
 
+__device__ __forceinline__ double energy_lam(const EnergyEIO& a, int64_t l) {
+    return a.lam ? a.lam[l] : lambda_eff(a.m[l], a.Sw[l], a.Sn[l], a.Sg[l], a.T[l]);
+}
+
 // v[0] = lambda_eff * T, v[1 + a] = (ro_a H_a) ux_a, v[4 + a] = (ro_a H_a) uy_a
 __device__ __forceinline__ void energy_e_vals(const EnergyEIO& a, int64_t l, double* v) {
-    v[0] = a.lam[l] * a.T[l];
+    v[0] = energy_lam(a, l) * a.T[l];
 #pragma unroll
     for (int ph = 0; ph < 3; ++ph) {
         const double rh = a.ro[ph][l] * a.H[ph][l];
@@ -466,7 +473,7 @@ __device__ __forceinline__ void energy_e_vals(const EnergyEIO& a, int64_t l, dou
 
 // lambda_eff * T and the uy products of a z neighbour: w[0], w[1 + a]
 __device__ __forceinline__ void energy_e_zvals(const EnergyEIO& a, int64_t l, double* w) {
-    w[0] = a.lam[l] * a.T[l];
+    w[0] = energy_lam(a, l) * a.T[l];
 #pragma unroll
     for (int ph = 0; ph < 3; ++ph) w[1 + ph] = a.ro[ph][l] * a.H[ph][l] * a.uy[ph][l];
 }
@@ -521,7 +528,7 @@ __global__ void __launch_bounds__(TX * TY) k_energy_e_z(DevParams p, EnergyEIO a
         __syncthreads();
         if (own_in_grid && hasz) energy_e_zvals(a, l + p.sz, zp);
         if (active) {
-            const double lc = a.lam[l], Tc = a.T[l];
+            const double lc = energy_lam(a, l), Tc = a.T[l];
             double T_flow = 0;
             if (p.Nx > 2)
                 T_flow += exact_div(lds[0][ty + 1][tx + 2] - 2 * lc * Tc + lds[0][ty + 1][tx], p.hx2, p.r_hx2);

@@ -274,6 +274,33 @@ hipError_t launch_roS_tiles(const DevParams& p, const DevFields& F, const double
     return hipGetLastError();
 }
 
+// E_new of the interior cells on the z tiles; lam: the step's stored
+// lambda_eff, or NULL to evaluate it per staged cell (the entry point).
+hipError_t launch_energy_e_tiles(const DevParams& p, const DevFields& F, const double* lam, hipStream_t s) {
+    EnergyEIO ea{};
+    ea.lam = lam;
+    ea.T = F.f[F_T];
+    ea.E = F.f[F_E];
+    ea.E_new = F.f[F_E_new];
+    ea.m = F.f[F_m];
+    ea.Sw = F.f[F_S_w];
+    ea.Sn = F.f[F_S_n];
+    ea.Sg = F.f[F_S_g];
+    const int ro[3] = {F_ro_w, F_ro_n, F_ro_g}, H[3] = {F_H_w, F_H_n, F_H_g};
+    const int ux[3] = {F_ux_w, F_ux_n, F_ux_g}, uy[3] = {F_uy_w, F_uy_n, F_uy_g};
+    for (int a = 0; a < 3; ++a) {
+        ea.ro[a] = F.f[ro[a]];
+        ea.H[a] = F.f[H[a]];
+        ea.ux[a] = F.f[ux[a]];
+        ea.uy[a] = F.f[uy[a]];
+    }
+    const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
+    const int kc = fused_kchunk(p, 0, 8, resident_groups<k_energy_e_z<64, 8>>(512));
+    dim3 ge((nx + 63) / 64, (ny + 7) / 8, (nz + kc - 1) / kc);
+    hipLaunchKernelGGL((k_energy_e_z<64, 8>), ge, dim3(64, 8), 0, s, p, ea, kc);
+    return hipGetLastError();
+}
+
 hipError_t launch_roS_stored(const DevParams& p, const DevFields& F, const double* dtm, hipStream_t s) {
     if (p.model != 3) return hipErrorInvalidValue;
     return launch_roS_tiles<false>(p, F, dtm, s);
@@ -288,26 +315,7 @@ hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* la
     // u of the boundary cells here, of the interior cells in the roS tiles
     hipLaunchKernelGGL(k_energy_u_faces, grid_for(fused_face_cells(p)), b, 0, s, p, F);
     if ((e = launch_roS_tiles<true>(p, F, dtm, s)) != hipSuccess) return e;
-    {
-        // E_new on the z tiles
-        EnergyEIO ea{};
-        ea.lam = lam;
-        ea.T = F.f[F_T];
-        ea.E = F.f[F_E];
-        ea.E_new = F.f[F_E_new];
-        const int ro[3] = {F_ro_w, F_ro_n, F_ro_g}, H[3] = {F_H_w, F_H_n, F_H_g};
-        const int ux[3] = {F_ux_w, F_ux_n, F_ux_g}, uy[3] = {F_uy_w, F_uy_n, F_uy_g};
-        for (int a = 0; a < 3; ++a) {
-            ea.ro[a] = F.f[ro[a]];
-            ea.H[a] = F.f[H[a]];
-            ea.ux[a] = F.f[ux[a]];
-            ea.uy[a] = F.f[uy[a]];
-        }
-        const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
-        const int kc = fused_kchunk(p, 0, 8, resident_groups<k_energy_e_z<64, 8>>(512));
-        dim3 ge((nx + 63) / 64, (ny + 7) / 8, (nz + kc - 1) / kc);
-        hipLaunchKernelGGL((k_energy_e_z<64, 8>), ge, dim3(64, 8), 0, s, p, ea, kc);
-    }
+    if ((e = launch_energy_e_tiles(p, F, lam, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_newton, g, b, 0, s, p, F);
     hipLaunchKernelGGL(k_boundary_faces, grid_for(fused_face_cells(p)), b, 0, s, p, F);
     if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -45,6 +45,9 @@ hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, h
 // from the P, Xi and ro arrays P_Xi_ro wrote (no cell may have m == 0: the
 // per-cell entry point keeps their stale roS); dtm holds dt / m per cell.
 hipError_t launch_roS_stored(const DevParams& p, const DevFields& F, const double* dtm, hipStream_t s);
+// E_new of the interior cells on z tiles (energy model); lam: lambda_eff per
+// cell computed earlier in the step, or NULL to evaluate it from the fields.
+hipError_t launch_energy_e_tiles(const DevParams& p, const DevFields& F, const double* lam, hipStream_t s);
 // One step of the energy model in six passes, in place on F; lam is a work
 // array of Nx*Ny*Nz doubles, dtm holds dt / m per cell; ev brackets the whole
 // step.
