@@ -210,6 +210,7 @@ int capaz_create(const capaz_params* u, capaz_ctx** out) {
     const size_t bytes = (size_t)c->p.n * sizeof(double);
     if ((e = hipMalloc(&c->d_dtm, bytes)) != hipSuccess) return bail(e, "hipMalloc");
     if (u->energy && (e = hipMalloc(&c->d_lam, bytes)) != hipSuccess) return bail(e, "hipMalloc");
+    if (u->energy && (e = hipMalloc(&c->d_redo, (size_t)c->p.n)) != hipSuccess) return bail(e, "hipMalloc");
     for (int f = 0; f < capaz::F_COUNT; ++f) {
         if (!field_exists(*u, f)) continue;
         if ((e = hipMalloc(&c->cur[f], bytes)) != hipSuccess) return bail(e, "hipMalloc");
@@ -236,6 +237,7 @@ int capaz_destroy(capaz_ctx* c) {
     if (c->d_flag) (void)hipFree(c->d_flag);
     if (c->d_dtm) (void)hipFree(c->d_dtm);
     if (c->d_lam) (void)hipFree(c->d_lam);
+    if (c->d_redo) (void)hipFree(c->d_redo);
     if (c->adj_mem) (void)hipFree(c->adj_mem);
     for (auto& e : c->ev) {
         if (e.begin) (void)hipEventDestroy(e.begin);
@@ -320,7 +322,16 @@ int capaz_energy_E(capaz_ctx* c) {
     CAPAZ_HIP(c, capaz::launch_energy_e_tiles(c->p, fields_of(c), nullptr, c->stream));
     return CAPAZ_OK;
 }
-CAPAZ_ENTRY(capaz_newton, Newton, true)
+int capaz_newton(capaz_ctx* c) {
+    int rc;
+    if ((rc = check_ctx(c))) return rc;
+    if ((rc = materialize(c))) return rc;
+    if (c->user.energy) {
+        CAPAZ_HIP(c, capaz::launch_newton_energy(c->p, fields_of(c), c->d_redo, c->stream));
+        return CAPAZ_OK;
+    }
+    return run_entry(c, capaz::Entry::Newton, fields_of(c));
+}
 CAPAZ_ENTRY(capaz_boundary, Boundary, true)
 
 int capaz_roS(capaz_ctx* c, double t) {
@@ -365,7 +376,7 @@ int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
         for (long s = 0; s < nsteps; ++s) {
             const KernelEvents* ev;
             if ((rc = next_events(c, &ev))) return rc;
-            CAPAZ_HIP(c, launch_energy_step(c->p, fields_of(c), c->d_lam, c->d_dtm, c->stream, ev));
+            CAPAZ_HIP(c, launch_energy_step(c->p, fields_of(c), c->d_lam, c->d_dtm, c->d_redo, c->stream, ev));
         }
         return CAPAZ_OK;
     }

@@ -229,6 +229,30 @@ CAPAZ_HD void body_E_with(const DevParams& p, const DevFields& F, const Lam& lam
 CAPAZ_HD void body_E(const DevParams& p, const DevFields& F, int64_t l) { body_E_with(p, F, LamOfFields{F}, l); }
 
 // ---- P_S_calculation -> Newton (cpu.cpp:80-89), interior cells -------------------------
+// The energy model's Newton update of interior cell l without the pivot
+// search: writes the new state and returns true, or writes nothing and
+// returns false when a pivot needs the search (body_newton then does the cell).
+CAPAZ_HD bool body_newton_energy_nopivot(const DevParams& p, const DevFields& F, int64_t l) {
+    double P = F.f[F_P_w][l], Sw = F.f[F_S_w][l], Sn = F.f[F_S_n][l];
+    double Sg = F.f[F_S_g][l], T = F.f[F_T][l];
+    double Hw = F.f[F_H_w][l], Hn = F.f[F_H_n][l], Hg = F.f[F_H_g][l], Hr = F.f[F_H_r][l];
+    const double E_new = F.f[F_E_new][l];
+    if (!newton_energy_t<true>(p, Sw, Sn, Sg, P, T, Hw, Hn, Hg, Hr, F.f[F_m][l], F.f[F_roS_w][l], F.f[F_roS_n][l],
+                               F.f[F_roS_g][l], E_new))
+        return false;
+    F.f[F_S_g][l] = Sg;
+    F.f[F_T][l] = T;
+    F.f[F_H_w][l] = Hw;
+    F.f[F_H_n][l] = Hn;
+    F.f[F_H_g][l] = Hg;
+    F.f[F_H_r][l] = Hr;
+    F.f[F_E][l] = E_new;
+    F.f[F_P_w][l] = P;
+    F.f[F_S_w][l] = Sw;
+    F.f[F_S_n][l] = Sn;
+    return true;
+}
+
 CAPAZ_HD void body_newton(const DevParams& p, const DevFields& F, int64_t l) {
     int i, j, k;
     decompose(p, l, i, j, k);

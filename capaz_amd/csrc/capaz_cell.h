@@ -477,11 +477,54 @@ CAPAZ_HD void mult_matrix_vector(double* r, const double* m, const double* v) {
     }
 }
 
+// reverse_matrix<N> when no pivot search is needed: the general routine's
+// operations in its order, without the row-swap code (whose data-dependent
+// swaps keep every row live across selects).  Returns false, leaving `a`
+// untouched, if some pivot has |f| < 1e-8.
+template <int N>
+CAPAZ_HD bool reverse_nopivot(double* a) {
+    constexpr int N1 = 2 * N;
+    const double e = 1E-8;
+    double m[N][N1];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+#pragma unroll
+        for (int j = 0; j < N; j++) m[i][j] = a[j + N * i];
+#pragma unroll
+        for (int j = N; j < N1; j++) m[i][j] = (j == N + i) ? 1. : 0.;
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const double f = m[k][k];
+        if (fabs(f) < e) return false;
+        const double rf = rcp_rn(f);  // RN(1/f), as the general routine's 1. / f
+#pragma unroll
+        for (int j = 0; j < N1; j++) m[k][j] = exact_div(m[k][j], f, rf);
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            if (i != k) {
+                const double f1 = m[i][k];
+#pragma unroll
+                for (int j = 0; j < N1; j++) m[i][j] -= (m[k][j] * f1);
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++)
+#pragma unroll
+        for (int j = 0; j < N; j++) a[j + N * i] = m[i][j + N];
+    return true;
+}
+
 // Energy-model 5x5 Newton in (S_w, S_n, S_g, P_w, T) (energy.cpp:370-463);
-// H_* are refreshed after every iteration as assign_H does.
-CAPAZ_HD void newton_energy(const DevParams& p, double& Sw, double& Sn, double& Sg, double& P, double& T, double& Hw,
-                            double& Hn, double& Hg, double& Hr, double m, double roS_w, double roS_n, double roS_g,
-                            double E_new) {
+// H_* are refreshed after every iteration as assign_H does.  NOPIVOT: every
+// inverse by reverse_nopivot; returns false as soon as one needs the pivot
+// search (the arguments are then partly updated: the caller keeps copies and
+// reruns the cell with NOPIVOT false, which always returns true).
+template <bool NOPIVOT>
+CAPAZ_HD bool newton_energy_t(const DevParams& p, double& Sw, double& Sn, double& Sg, double& P, double& T, double& Hw,
+                              double& Hn, double& Hg, double& Hr, double m, double roS_w, double roS_n, double roS_g,
+                              double E_new) {
     constexpr int n = 5;
     for (int w = 1; w <= p.iterations; w++) {
         double rw = ro_w_T(p, P, T), rn = ro_n_T(p, P, T), rg = ro_g_T(p, P, T);
@@ -506,7 +549,11 @@ CAPAZ_HD void newton_energy(const DevParams& p, double& Sw, double& Sn, double& 
         dF[4 + n * 4] = m * (Sw * (dwT * Hw + rw * c_w_T(T)) + Sn * (dnT * Hn + rn * c_n_T(T)) +
                              Sg * (dgT * Hg + rg * c_g_T(T))) +
                         (1. - m) * kRoR * c_r_T(T);
-        reverse_matrix<5>(dF);
+        if constexpr (NOPIVOT) {
+            if (!reverse_nopivot<5>(dF)) return false;
+        } else {
+            reverse_matrix<5>(dF);
+        }
         mult_matrix_vector<5>(corr, dF, F);
         Sw = Sw - corr[0];
         Sn = Sn - corr[1];
@@ -518,6 +565,13 @@ CAPAZ_HD void newton_energy(const DevParams& p, double& Sw, double& Sn, double& 
         Hg = H_g_T(p, T);
         Hr = H_r_T(p, T);
     }
+    return true;
+}
+
+CAPAZ_HD void newton_energy(const DevParams& p, double& Sw, double& Sn, double& Sg, double& P, double& T, double& Hw,
+                            double& Hn, double& Hg, double& Hr, double m, double roS_w, double roS_n, double roS_g,
+                            double E_new) {
+    newton_energy_t<false>(p, Sw, Sn, Sg, P, T, Hw, Hn, Hg, Hr, m, roS_w, roS_n, roS_g, E_new);
 }
 
 // ---- three-phase Border_P faces (three-phase.cpp:383-422) ---------------------------------

@@ -23,6 +23,7 @@ struct capaz_ctx {
     int* d_flag = nullptr;
     double* d_dtm = nullptr;  // dt / m per cell, refreshed with the porosity scan
     double* d_lam = nullptr;  // energy model: lambda_eff per cell within a step
+    unsigned char* d_redo = nullptr;  // energy model: cells the Newton pass redoes with pivoting
     capaz::Tuning tuning;
     // Event timing of the fused step's main kernel (capaz_tune "timing"):
     // a pool of event pairs, drained into the totals when it fills or is read.

@@ -61,6 +61,22 @@ __global__ void __launch_bounds__(kBlock) k_energy_u_faces(DevParams p, DevField
     if (face_cell(p, (int64_t)blockIdx.x * kBlock + threadIdx.x, i, j, k))
         body_u(p, F, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz);
 }
+// The energy model's Newton update: every interior cell without the pivot
+// search (166 VGPRs, no scratch: 3 waves per SIMD against 2 for the general
+// routine); a cell whose matrices need the search is marked in `redo` and
+// recomputed from its untouched state by k_newton_energy_redo.
+__global__ void __launch_bounds__(kBlock) k_newton_energy(DevParams p, DevFields F, unsigned char* redo) {
+    const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l >= p.n) return;
+    int i, j, k;
+    decompose(p, l, i, j, k);
+    redo[l] = interior3(p, i, j, k) && !body_newton_energy_nopivot(p, F, l) ? 1 : 0;
+}
+__global__ void __launch_bounds__(kBlock) k_newton_energy_redo(DevParams p, DevFields F, const unsigned char* redo) {
+    const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (l < p.n && redo[l]) body_newton(p, F, l);
+}
+
 // boundary_conditions over the boundary cells only (the entry point and the
 // energy step's last pass)
 __global__ void __launch_bounds__(kBlock) k_boundary_faces(DevParams p, DevFields F) {
@@ -301,13 +317,19 @@ hipError_t launch_energy_e_tiles(const DevParams& p, const DevFields& F, const d
     return hipGetLastError();
 }
 
+hipError_t launch_newton_energy(const DevParams& p, const DevFields& F, unsigned char* redo, hipStream_t s) {
+    hipLaunchKernelGGL(k_newton_energy, grid_for(p.n), dim3(kBlock), 0, s, p, F, redo);
+    hipLaunchKernelGGL(k_newton_energy_redo, grid_for(p.n), dim3(kBlock), 0, s, p, F, redo);
+    return hipGetLastError();
+}
+
 hipError_t launch_roS_stored(const DevParams& p, const DevFields& F, const double* dtm, hipStream_t s) {
     if (p.model != 3) return hipErrorInvalidValue;
     return launch_roS_tiles<false>(p, F, dtm, s);
 }
 
-hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* lam, const double* dtm, hipStream_t s,
-                              const KernelEvents* ev) {
+hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* lam, const double* dtm,
+                              unsigned char* redo, hipStream_t s, const KernelEvents* ev) {
     hipError_t e;
     const dim3 g = grid_for(p.n), b(kBlock);
     if (ev && (e = hipEventRecord(ev->begin, s)) != hipSuccess) return e;
@@ -316,7 +338,7 @@ hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* la
     hipLaunchKernelGGL(k_energy_u_faces, grid_for(fused_face_cells(p)), b, 0, s, p, F);
     if ((e = launch_roS_tiles<true>(p, F, dtm, s)) != hipSuccess) return e;
     if ((e = launch_energy_e_tiles(p, F, lam, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL(k_newton, g, b, 0, s, p, F);
+    if ((e = launch_newton_energy(p, F, redo, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(k_boundary_faces, grid_for(fused_face_cells(p)), b, 0, s, p, F);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     return ev ? hipEventRecord(ev->end, s) : hipSuccess;

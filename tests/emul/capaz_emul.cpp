@@ -6,6 +6,7 @@
 // only difference from the device build is the math library behind pow/exp
 // (glibc here, the ROCm device library on the GPU).
 #include <cstdint>
+#include <vector>
 
 #include "capaz_adjoint.h"
 #include "capaz_bodies.h"
@@ -19,12 +20,26 @@ const char* capaz_emul_validate(const capaz_params* u) {
     return why.c_str();
 }
 
-// entry: 0 S, 1 P_Xi_ro, 2 u, 3 roS, 4 HE, 5 E, 6 Newton, 7 boundary.
+// entry: 0 S, 1 P_Xi_ro, 2 u, 3 roS, 4 HE, 5 E, 6 Newton, 7 boundary;
+// 8 the energy model's Newton in its two passes (returns the cells redone).
 int capaz_emul_entry(const capaz_params* u, double** fields, int entry) {
     if (!capaz::validate(*u).empty()) return -1;
     capaz::DevParams p = capaz::derive(*u);
     capaz::DevFields F;
     for (int f = 0; f < capaz::F_COUNT; ++f) F.f[f] = fields[f];
+    if (entry == 8) {  // the energy Newton as the kernels run it: no-pivot pass, then the redo pass
+        if (!p.energy) return -1;
+        std::vector<unsigned char> redo(p.n);
+        int redone = 0;
+        for (int64_t l = 0; l < p.n; ++l) {
+            int i, j, k;
+            capaz::decompose(p, l, i, j, k);
+            redo[l] = capaz::interior3(p, i, j, k) && !capaz::body_newton_energy_nopivot(p, F, l);
+        }
+        for (int64_t l = 0; l < p.n; ++l)
+            if (redo[l]) capaz::body_newton(p, F, l), ++redone;
+        return redone;
+    }
     for (int64_t l = 0; l < p.n; ++l) {
         switch (entry) {
             case 0: capaz::body_S(p, F, l); break;
@@ -108,6 +123,7 @@ int capaz_emul_reverse_matrix(double* a, int n) {
     if (n == -4) return capaz::reverse3<true>(a);  // the same for a[2] == +0
     if (n == 3) return capaz::reverse_matrix<3>(a);
     if (n == 5) return capaz::reverse_matrix<5>(a);
+    if (n == -5) return capaz::reverse_nopivot<5>(a) ? 0 : 1;  // 1: refused, `a` untouched
     return -2;
 }
 

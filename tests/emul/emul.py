@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libcapaz_emul.so")
 
 ENTRY = {"S_calculation": 0, "ro_P_Xi_calculation": 1, "u_calculation": 2, "roS_calculation": 3,
-         "H_E_current_calculation": 4, "E_calculation": 5, "P_S_calculation": 6, "boundary_conditions": 7}
+         "H_E_current_calculation": 4, "E_calculation": 5, "P_S_calculation": 6, "boundary_conditions": 7,
+         "newton_energy_two_pass": 8}
 
 _lib = None
 
@@ -49,10 +50,11 @@ def validate(keys: dict) -> str:
 
 def reverse_matrix(a: np.ndarray, fast: bool = False, a02zero: bool = False) -> tuple[int, np.ndarray]:
     """The kernels' reverse_matrix<n>; fast=True runs the 3x3 no-pivot path
-    with its fallback, a02zero=True its variant for a[0,2] == +0 (the one
-    newton3 uses)."""
+    with its fallback (for n = 5 the no-pivot routine alone, rc 1 when it
+    refuses), a02zero=True its variant for a[0,2] == +0 (the one newton3
+    uses)."""
     buf = np.ascontiguousarray(a, dtype=np.float64).copy()
-    n = -4 if a02zero else (-3 if fast else a.shape[0])
+    n = -4 if a02zero else (-a.shape[0] if fast else a.shape[0])
     rc = lib().capaz_emul_reverse_matrix(buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
     return rc, buf
 
@@ -101,8 +103,9 @@ class EmulDomain:
             self.call("boundary_conditions")
             return
         rc = lib().capaz_emul_entry(ctypes.byref(self.p), self._table(), ENTRY[entry])
-        if rc:
+        if rc < 0 or (rc and entry != "newton_energy_two_pass"):
             raise RuntimeError(f"emul entry {entry} failed")
+        self.redone = rc
 
     def run(self, nsteps: int, first_step: int = 1) -> None:
         for s in range(first_step, first_step + nsteps):

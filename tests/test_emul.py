@@ -59,3 +59,51 @@ def test_reverse3_a02zero_path_matches_oracle():
         assert bit_equal(out_o, out_e), a
         hits += rc_o == 0
     assert hits > 10000
+
+
+def test_reverse5_nopivot_matches_oracle_or_refuses():
+    """The energy Newton's 5x5 inverse without the pivot search: the oracle's
+    bits whenever every pivot has |f| >= 1e-8, otherwise it refuses and leaves
+    the matrix as it was (the kernel then redoes the cell with the search)."""
+    hits = refused = 0
+    for a in random_matrices(5, 5000, 11):
+        rc_o, out_o = oracle.reverse_matrix(a)
+        rc_e, out_e = emul.reverse_matrix(a, fast=True)
+        if rc_e == 1:
+            refused += 1
+            assert bit_equal(out_e, a)
+        else:
+            assert rc_o == 0 and bit_equal(out_o, out_e), a
+            hits += 1
+    assert hits > 2000 and refused > 1000
+
+
+@pytest.mark.parametrize("closure,gas_free", [("reference", False), ("corey", False), ("reference", True)])
+def test_energy_newton_two_passes_equal_the_general_routine(closure, gas_free):
+    """The energy model's Newton as the kernels run it (every interior cell
+    without the pivot search, then the refused cells again with it) leaves
+    every field with the bits of the general routine.  gas_free: water and
+    oil incompressible and a few cells with S_w + S_n = 1, whose pressure
+    pivot is then 0, so the second pass runs."""
+    from capaz_amd import workloads
+    keys, st = workloads.energy_3ph(12, 10, 8, closure=closure)
+    st = {f: np.array(v, copy=True) for f, v in st.items()}
+    if gas_free:
+        keys = dict(keys, BETA_W=0.0, BETA_N=0.0)
+        for c in ((3, 4, 5), (4, 5, 6), (5, 2, 8)):
+            st["S_w"][c], st["S_n"][c] = 0.625, 0.375
+    a, b = emul.EmulDomain(keys, st), emul.EmulDomain(keys, st)
+    redone = 0
+    for _ in range(6):
+        for d in (a, b):
+            for e in ("S_calculation", "ro_P_Xi_calculation", "u_calculation", "roS_calculation",
+                      "H_E_current_calculation", "E_calculation"):
+                d.call(e)
+        a.call("P_S_calculation")
+        b.call("newton_energy_two_pass")
+        redone += b.redone
+        for d in (a, b):
+            d.call("boundary_conditions")
+    assert redone >= 3 if gas_free else redone == 0
+    for f, v in a.fields().items():
+        assert bit_equal(b.field(f), v), f

@@ -151,6 +151,33 @@ def test_energy_workload_vs_oracle():
     check_fields(d.fields(), od.fields(), TOL_RUN, "energy after 6 steps")
 
 
+def test_energy_pivot_search_cells_vs_oracle():
+    """Water and oil incompressible and four cells with S_w + S_n = 1: their
+    5x5 Newton matrices have a zero pressure pivot, so the Newton pass leaves
+    them to its second kernel (the general routine with the pivot search;
+    here it finds none, and the update turns H_* non-finite as the oracle's
+    does).  One step: the same non-finite cells, every other value within
+    1e-12 of the oracle."""
+    keys, st = workloads.energy_3ph(20, 18, 14)
+    keys = dict(keys, BETA_W=0.0, BETA_N=0.0)
+    st = {f: np.array(v, copy=True) for f, v in st.items()}
+    for c in ((3, 4, 5), (4, 5, 6), (5, 2, 8), (10, 10, 10)):
+        st["S_w"][c], st["S_n"][c] = 0.625, 0.375
+    od = oracle.OracleDomain(keys, st)
+    d = capaz.Domain(keys, st)
+    od.run(1)
+    d.run(1)
+    bad = 0
+    for f, w in od.fields().items():
+        g = d.field(f)
+        fin = np.isfinite(w)
+        assert np.array_equal(fin, np.isfinite(g)), f
+        bad += int((~fin).sum())
+        assert max_rel(g[fin], w[fin]) <= TOL_STEP1, (f, max_rel(g[fin], w[fin]))
+    assert bad == 16
+    d.close()
+
+
 def test_full_size_fused_equals_sequence():
     """256^3 headline workload: two fused steps and two entry-point sequences
     give identical bits in every field."""
