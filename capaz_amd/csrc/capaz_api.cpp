@@ -210,7 +210,8 @@ int capaz_create(const capaz_params* u, capaz_ctx** out) {
     const size_t bytes = (size_t)c->p.n * sizeof(double);
     if ((e = hipMalloc(&c->d_dtm, bytes)) != hipSuccess) return bail(e, "hipMalloc");
     if (u->energy && (e = hipMalloc(&c->d_lam, bytes)) != hipSuccess) return bail(e, "hipMalloc");
-    if (u->energy && (e = hipMalloc(&c->d_redo, (size_t)c->p.n)) != hipSuccess) return bail(e, "hipMalloc");
+    if (u->energy && (e = hipMalloc(&c->d_redo, ((size_t)c->p.n + 15) / 16 * 16)) != hipSuccess)
+        return bail(e, "hipMalloc");
     for (int f = 0; f < capaz::F_COUNT; ++f) {
         if (!field_exists(*u, f)) continue;
         if ((e = hipMalloc(&c->cur[f], bytes)) != hipSuccess) return bail(e, "hipMalloc");

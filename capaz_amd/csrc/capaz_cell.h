@@ -478,9 +478,13 @@ CAPAZ_HD void mult_matrix_vector(double* r, const double* m, const double* v) {
 }
 
 // reverse_matrix<N> when no pivot search is needed: the general routine's
-// operations in its order, without the row-swap code (whose data-dependent
-// swaps keep every row live across selects).  Returns false, leaving `a`
-// untouched, if some pivot has |f| < 1e-8.
+// operations in its order on the columns that can still reach the result.
+// At step k those are the left columns after k and the right (inverse)
+// columns up to N + k: the left columns up to k are never read again, and a
+// right column after N + k is still the identity's (zero outside row k's
+// divided zero), so the skipped updates only subtract zeros.  The result has
+// the general routine's bits up to the sign of entries that are exactly 0.
+// Returns false, leaving `a` untouched, if some pivot has |f| < 1e-8.
 template <int N>
 CAPAZ_HD bool reverse_nopivot(double* a) {
     constexpr int N1 = 2 * N;
@@ -499,13 +503,13 @@ CAPAZ_HD bool reverse_nopivot(double* a) {
         if (fabs(f) < e) return false;
         const double rf = rcp_rn(f);  // RN(1/f), as the general routine's 1. / f
 #pragma unroll
-        for (int j = 0; j < N1; j++) m[k][j] = exact_div(m[k][j], f, rf);
+        for (int j = k + 1; j <= N + k; j++) m[k][j] = exact_div(m[k][j], f, rf);
 #pragma unroll
         for (int i = 0; i < N; i++) {
             if (i != k) {
                 const double f1 = m[i][k];
 #pragma unroll
-                for (int j = 0; j < N1; j++) m[i][j] -= (m[k][j] * f1);
+                for (int j = k + 1; j <= N + k; j++) m[i][j] -= (m[k][j] * f1);
             }
         }
     }

@@ -72,9 +72,16 @@ __global__ void __launch_bounds__(kBlock) k_newton_energy(DevParams p, DevFields
     decompose(p, l, i, j, k);
     redo[l] = interior3(p, i, j, k) && !body_newton_energy_nopivot(p, F, l) ? 1 : 0;
 }
+// One thread per 16 marks (one 16-byte load; redo is allocated rounded up to
+// 16 bytes, the marks past Nx*Ny*Nz are never read as cells).
 __global__ void __launch_bounds__(kBlock) k_newton_energy_redo(DevParams p, DevFields F, const unsigned char* redo) {
-    const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (l < p.n && redo[l]) body_newton(p, F, l);
+    const int64_t l0 = ((int64_t)blockIdx.x * kBlock + threadIdx.x) * 16;
+    if (l0 >= p.n) return;
+    const uint4 w = *reinterpret_cast<const uint4*>(redo + l0);
+    if ((w.x | w.y | w.z | w.w) == 0u) return;
+    const int64_t end = l0 + 16 < p.n ? l0 + 16 : p.n;
+    for (int64_t l = l0; l < end; ++l)
+        if (redo[l]) body_newton(p, F, l);
 }
 
 // boundary_conditions over the boundary cells only (the entry point and the
@@ -319,7 +326,7 @@ hipError_t launch_energy_e_tiles(const DevParams& p, const DevFields& F, const d
 
 hipError_t launch_newton_energy(const DevParams& p, const DevFields& F, unsigned char* redo, hipStream_t s) {
     hipLaunchKernelGGL(k_newton_energy, grid_for(p.n), dim3(kBlock), 0, s, p, F, redo);
-    hipLaunchKernelGGL(k_newton_energy_redo, grid_for(p.n), dim3(kBlock), 0, s, p, F, redo);
+    hipLaunchKernelGGL(k_newton_energy_redo, grid_for((p.n + 15) / 16), dim3(kBlock), 0, s, p, F, redo);
     return hipGetLastError();
 }
 

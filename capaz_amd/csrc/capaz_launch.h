@@ -50,12 +50,12 @@ hipError_t launch_roS_stored(const DevParams& p, const DevFields& F, const doubl
 hipError_t launch_energy_e_tiles(const DevParams& p, const DevFields& F, const double* lam, hipStream_t s);
 // One step of the energy model in six passes, in place on F; lam is a work
 // array of Nx*Ny*Nz doubles, dtm holds dt / m per cell, redo Nx*Ny*Nz bytes
-// for the Newton pass; ev brackets the whole step.
+// (rounded up to 16) for the Newton pass; ev brackets the whole step.
 hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* lam, const double* dtm,
                               unsigned char* redo, hipStream_t s, const KernelEvents* ev = nullptr);
 // The energy model's Newton entry point: interior cells without the pivot
-// search, then the cells that need it (marked in redo, Nx*Ny*Nz bytes) with
-// the general routine.
+// search, then the cells that need it (marked in redo, Nx*Ny*Nz bytes rounded
+// up to 16) with the general routine.
 hipError_t launch_newton_energy(const DevParams& p, const DevFields& F, unsigned char* redo, hipStream_t s);
 // Planes per tile of the fused z kernels: kchunk if >= 1, else chosen from
 // the grid and the number of workgroups resident at once, so that small grids
