@@ -55,9 +55,30 @@ __global__ void k_div(uint64_t n, uint64_t seed, int emin, int emax, unsigned lo
     if (local) atomicAdd(bad, local);
 }
 
+__global__ void k_map(int which, const double* x, double* y, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = which == 0 ? capaz::pow_half(x[i]) : capaz::pow_two(x[i]);
+}
+
 }  // namespace
 
 extern "C" {
+
+// y = pow_half(x) (which = 0) or pow_two(x) (which = 1) on the device for n
+// host doubles; 0, or -1 on a HIP error.
+int capaz_selftest_map(int which, const double* x, double* y, unsigned long long n) {
+    double *dx = nullptr, *dy = nullptr;
+    int rc = -1;
+    if (hipMalloc(&dx, n * sizeof(double)) == hipSuccess && hipMalloc(&dy, n * sizeof(double)) == hipSuccess &&
+        hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice) == hipSuccess) {
+        hipLaunchKernelGGL(k_map, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, which, dx, dy, (uint64_t)n);
+        if (hipGetLastError() == hipSuccess && hipMemcpy(y, dy, n * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess)
+            rc = 0;
+    }
+    if (dx) (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    return rc;
+}
 
 // Number of draws where rcp_rn(b) != 1.0/b (which = 0) or
 // exact_div(a, b, rcp_rn(b)) != a/b (which = 1); -1 on a HIP error.
