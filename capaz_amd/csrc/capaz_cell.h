@@ -135,9 +135,10 @@ CAPAZ_HD double pow_two(double t) {
 
 // van Genuchten-type k_r (three-phase.cpp:146-186); p.vg_a = A/(A-1.),
 // p.vg_b = (A-1.)/A, p.vg_c = 2.*(A-1.)/A.
-CAPAZ_HD double vg_k_w(const DevParams& p, double Swe) {
+// pSwe = pow(Swe, p.vg_a), shared by k_w and k_n (read only where they use it).
+CAPAZ_HD double vg_k_w(const DevParams& p, double Swe, double pSwe) {
     if (Swe >= 1e-3) {
-        double t = 1. - pow(1. - pow(Swe, p.vg_a), p.vg_b);
+        double t = 1. - pow(1. - pSwe, p.vg_b);
         return pow_half(Swe) * pow_two(t);
     }
     return 0;
@@ -146,10 +147,10 @@ CAPAZ_HD double vg_k_g(const DevParams& p, double Sge) {
     if (Sge >= 1e-3) return pow_half(Sge) * pow(1. - pow(1. - Sge, p.vg_a), p.vg_c);
     return 0;
 }
-CAPAZ_HD double vg_k_n(const DevParams& p, double Swe, double Sne) {
+CAPAZ_HD double vg_k_n(const DevParams& p, double Swe, double Sne, double pSwe) {
     double Sge = 1. - Swe - Sne;
     if (Sne >= 1e-3) {
-        double knw = pow_half(1. - Swe) * pow(1. - pow(Swe, p.vg_a), p.vg_c);
+        double knw = pow_half(1. - Swe) * pow(1. - pSwe, p.vg_c);
         double kng = pow_half(Sne) * pow_two(1. - pow(1. - pow(Sne, p.vg_a), p.vg_b));
         return Sne * knw * kng / (1 - Swe) / (1 - Sge);
     }
@@ -167,9 +168,10 @@ CAPAZ_HD void relperm3(const DevParams& p, double S_w, double S_n, double& kw, d
         kn = corey_kr(p, Sne);
         kg = corey_kr(p, Sge);
     } else {
-        kw = vg_k_w(p, Swe);
+        const double pSwe = Swe >= 1e-3 || Sne >= 1e-3 ? pow(Swe, p.vg_a) : 0.;
+        kw = vg_k_w(p, Swe, pSwe);
         kg = vg_k_g(p, Sge);
-        kn = vg_k_n(p, Swe, Sne);
+        kn = vg_k_n(p, Swe, Sne, pSwe);
     }
 }
 
