@@ -3,7 +3,8 @@
 filtration step at 256^3 on one MI355X, with the HBM roofline of the fused
 step kernel and the CPU oracle timed beside it.
 
-    python bench.py [--steps K] [--warmup W] [--n 256] [--no-cpu-baseline] [--no-ensemble] [--no-adjoint] [--no-energy]
+    python bench.py [--steps K] [--warmup W] [--n 256] [--no-cpu-baseline] [--no-ensemble] [--no-adjoint]
+                    [--no-energy] [--no-shard] [--only ensemble|adjoint|energy|shard]
 
 A step is one fused time step (capaz_step: closures, upwind Darcy fluxes,
 mass update, 4-iteration 3x3 Newton per cell and the boundary faces) over all
@@ -274,6 +275,58 @@ def bench_energy(nx: int = 256, ny: int = 256, nz: int = 128, steps: int = 10) -
                     "sample": f"{cpu_steps} steps of a {nx}x{ny}x{slab} slab, {cpu:.2f} s"}}
 
 
+def bench_shard(n: int = 256, gpus: int = 8, steps: int = 200, warmup: int = 20) -> dict:
+    """One GPU's share of the headline workload under a z-slab decomposition
+    over `gpus` GPUs, at its real shape: n x n x (n/gpus + 2) cells (the owned
+    planes and one ghost plane per neighbour) of the same physics and state,
+    `steps` fused steps timed with HIP events on the context's stream.  After
+    the timed steps, `steps` rounds of the halo's device-side pack and unpack
+    (capaz_load_face / capaz_save_face of P_w, S_w, S_n at both z faces, the
+    planes a neighbour would send) run untimed here, for a kernel trace to
+    time; the GPU-to-GPU transfer itself needs a second GPU and is not run."""
+    import ctypes
+    from capaz_amd import capaz, hip, workloads
+
+    nz = n // gpus + 2
+    keys, st = workloads.headline(n)
+    keys = dict(keys, NZ=nz)
+    st = {f: np.ascontiguousarray(v[:nz]) for f, v in st.items()}
+    d = capaz.Domain(keys, st)
+    del st
+    d.run(warmup, first_step=1)
+    d.sync()
+    stream = d.stream()
+    ev0, ev1 = hip.Event(), hip.Event()
+    ev0.record(stream)
+    d.run(steps, first_step=warmup + 1)
+    ev1.record(stream)
+    d.sync()
+    ms = ev0.elapsed_ms(ev1) / steps
+    L = capaz.load()
+    plane = n * n
+    bufs = [hip.DeviceBuffer(8 * plane) for _ in range(6)]
+    fids = [capaz.FIELD_ID[f] for f in ("P_w", "S_w", "S_n")]
+    for _ in range(steps):
+        for q, fid in enumerate(fids):
+            for side in (0, 1):
+                b = ctypes.c_void_p(bufs[2 * q + side].address)
+                assert L.capaz_load_face(d._ctx, fid, 2, side, b) == 0
+        for q, fid in enumerate(fids):
+            for side in (0, 1):
+                b = ctypes.c_void_p(bufs[2 * q + (1 - side)].address)
+                assert L.capaz_save_face(d._ctx, fid, 2, side, b) == 0
+    d.sync()
+    for b in bufs:
+        b.close()
+    d.close()
+    owned = n * n * (nz - 2)
+    return {"workload": f"one z-slab of the {n}^3 headline over {gpus} GPUs: {n}x{n}x{nz} "
+                        f"({nz - 2} owned + 2 ghost planes)",
+            "ms_per_step": round(ms, 4), "value": round(owned / (ms / 1e3) / 1e6, 1),
+            "unit": "Mcell-steps/s (owned cells)",
+            "halo_bytes_per_step": 2 * 3 * 8 * plane}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -284,7 +337,8 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-energy", action="store_true")
-    ap.add_argument("--only", choices=("ensemble", "adjoint", "energy"), default=None,
+    ap.add_argument("--no-shard", action="store_true")
+    ap.add_argument("--only", choices=("ensemble", "adjoint", "energy", "shard"), default=None,
                     help="run only this secondary measurement and print it")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a copy of known size, to scale "
@@ -305,7 +359,8 @@ def main() -> int:
         return 2
     if args.only:
         # one secondary measurement alone (for profiling), as its own JSON line
-        fn = {"ensemble": bench_ensemble, "adjoint": bench_adjoint, "energy": bench_energy}[args.only]
+        fn = {"ensemble": bench_ensemble, "adjoint": bench_adjoint, "energy": bench_energy,
+              "shard": bench_shard}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
         return 0
 
@@ -383,6 +438,9 @@ def main() -> int:
     if not args.no_energy:
         print("bench: energy", file=sys.stderr, flush=True)
         out["energy"] = bench_energy()
+    if not args.no_shard:
+        print("bench: one z-slab shard", file=sys.stderr, flush=True)
+        out["shard"] = bench_shard(args.n)
     if not args.no_cpu_baseline:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         cb = cpu_baseline(args.n)
