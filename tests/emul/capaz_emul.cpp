@@ -36,8 +36,11 @@ int capaz_emul_entry(const capaz_params* u, double** fields, int entry) {
             capaz::decompose(p, l, i, j, k);
             redo[l] = capaz::interior3(p, i, j, k) && !capaz::body_newton_energy_nopivot(p, F, l);
         }
-        for (int64_t l = 0; l < p.n; ++l)
-            if (redo[l]) capaz::body_newton(p, F, l), ++redone;
+        for (int64_t l = 0; l < p.n; ++l) {
+            if (!redo[l]) continue;
+            capaz::body_newton(p, F, l);
+            ++redone;
+        }
         return redone;
     }
     for (int64_t l = 0; l < p.n; ++l) {
