@@ -125,6 +125,18 @@ __device__ __forceinline__ bool inner_cell(const DevParams& p, int i, int j, int
     return MODEL == 2 ? interior_newton2(p, i, j, k) : interior3(p, i, j, k);
 }
 
+// inner_cell split into its (i, j) and k factors: inner_cell(p, i, j, k) ==
+// inner_ij(p, i, j) && inner_k(p, k).  A tile column evaluates the (i, j)
+// factor once, so only one lane mask per column stays live across the planes.
+template <int MODEL>
+__device__ __forceinline__ bool inner_ij(const DevParams& p, int i, int j) {
+    const bool ii = (i != 0) && (i != p.Nx - 1);
+    return (MODEL == 2 ? ii : (ii || p.Nx < 2)) && (j != 0) && (j != p.Ny - 1);
+}
+__device__ __forceinline__ bool inner_k(const DevParams& p, int k) {
+    return ((k != 0) && (k != p.Nz - 1)) || (p.Nz < 2);
+}
+
 // Boundary cell (bi, bj, bk) from its inward neighbour's new state.
 template <int MODEL>
 __device__ __forceinline__ void write_boundary(const DevParams& p, const StepIO& io, int bi, int bj, int bk,
@@ -210,13 +222,14 @@ __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& i
 
     const int64_t col = (int64_t)i + (int64_t)j * p.sy;
     const int64_t hcol = (int64_t)hi + (int64_t)hj * p.sy;
+    const bool own_ij = inner_ij<MODEL>(p, i, j), halo_ij = inner_ij<MODEL>(p, hi, hj);
 
     CoefT<NPH> zm{}, c{}, zp{};
     double dz[NPH];
     if (own_in_grid) {
-        c = coef_at<MODEL, CLOSURE>(p, io, col + (int64_t)k0 * p.sz, inner_cell<MODEL>(p, i, j, k0));
+        c = coef_at<MODEL, CLOSURE>(p, io, col + (int64_t)k0 * p.sz, own_ij && inner_k(p, k0));
         if (hasz) {
-            zm = coef_at<MODEL, CLOSURE>(p, io, col + (int64_t)(k0 - 1) * p.sz, inner_cell<MODEL>(p, i, j, k0 - 1));
+            zm = coef_at<MODEL, CLOSURE>(p, io, col + (int64_t)(k0 - 1) * p.sz, own_ij && inner_k(p, k0 - 1));
 #pragma unroll
             for (int a = 0; a < NPH; ++a) dz[a] = exact_div(c.P[a] - zm.P[a], p.hz, p.r_hz);
         }
@@ -224,7 +237,7 @@ __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& i
 
     for (int k = k0; k <= k1; ++k) {
         if (!CARRY && own_in_grid && k != k0)
-            c = coef_at<MODEL, CLOSURE>(p, io, col + (int64_t)k * p.sz, inner_cell<MODEL>(p, i, j, k));
+            c = coef_at<MODEL, CLOSURE>(p, io, col + (int64_t)k * p.sz, own_ij && inner_k(p, k));
         // stage plane k: own cell and halo cell
         if (own_in_grid) {
 #pragma unroll
@@ -235,7 +248,7 @@ __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& i
             }
         }
         if (hi >= 0 && halo_in_grid) {
-            CoefT<NPH> h = coef_at<MODEL, CLOSURE>(p, io, hcol + (int64_t)k * p.sz, inner_cell<MODEL>(p, hi, hj, k));
+            CoefT<NPH> h = coef_at<MODEL, CLOSURE>(p, io, hcol + (int64_t)k * p.sz, halo_ij && inner_k(p, k));
 #pragma unroll
             for (int a = 0; a < NPH; ++a) {
                 lds.v[a][hy][hx] = h.P[a];
@@ -246,7 +259,7 @@ __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& i
         __syncthreads();
 
         if (own_in_grid && hasz)
-            zp = coef_at<MODEL, CLOSURE>(p, io, col + (int64_t)(k + 1) * p.sz, inner_cell<MODEL>(p, i, j, k + 1));
+            zp = coef_at<MODEL, CLOSURE>(p, io, col + (int64_t)(k + 1) * p.sz, own_ij && inner_k(p, k + 1));
         double dzp[NPH];
 #pragma unroll
         for (int a = 0; a < NPH; ++a) dzp[a] = hasz ? exact_div(zp.P[a] - c.P[a], p.hz, p.r_hz) : 0.;
