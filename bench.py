@@ -250,7 +250,7 @@ def bench_energy(nx: int = 256, ny: int = 256, nz: int = 128, steps: int = 10) -
     wall = timed(1, 1)
     wall_seq = timed(0, steps + 3)
     d.close()
-    # the same model with config 0's relative permeabilities (Corey-2)
+    # the same model with the Buckley-Leverett workload's relative permeabilities (Corey-2)
     keys_c, st_c = workloads.energy_3ph(nx, ny, nz, closure="corey")
     d = capaz.Domain(keys_c, st_c)
     wall_corey = timed(1, 1)
@@ -267,7 +267,7 @@ def bench_energy(nx: int = 256, ny: int = 256, nz: int = 128, steps: int = 10) -
             "value": round(cells * steps / wall / 1e6, 1), "unit": "Mcell-steps/s",
             "ms_per_step": round(wall * 1e3 / steps, 3),
             "entry_point_sequence_ms_per_step": round(wall_seq * 1e3 / steps, 3),
-            "corey": {"relperms": "Corey-2, S_wr 0.1, S_nr 0.05 (config 0's)",
+            "corey": {"relperms": "Corey-2, S_wr 0.1, S_nr 0.05 (the 40^3 Buckley-Leverett workload's)",
                       "value": round(cells * steps / wall_corey / 1e6, 1), "unit": "Mcell-steps/s",
                       "ms_per_step": round(wall_corey * 1e3 / steps, 3)},
             "cpu": {"value": round(nx * ny * slab * cpu_steps / cpu / 1e6, 2), "unit": "Mcell-steps/s",

@@ -164,7 +164,7 @@ def energy_3ph(nx: int = 256, ny: int = 256, nz: int = 128, seed: int = 1002, cl
     Initial state: hydrostatic water pressure, S = (0.4, 0.3, 0.3), T = 285 K
     (three-phase.cpp:470-487), porosity 0.4(1 + 0.05 N(0,1)) clipped to
     [0.3, 0.5], uniform K_0.  The reference has no two-phase energy model, so
-    this is its own energy model on the grid of the non-isothermal config.
+    this is its own energy model on the grid of the non-isothermal workload.
     closure="corey": the relative permeabilities of the two-phase workload
     instead (Corey-2, S_wr 0.1, S_nr 0.05), the rest unchanged."""
     rng = np.random.default_rng(seed)

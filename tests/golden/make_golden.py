@@ -6,8 +6,8 @@ Each fixture holds, for one deck and seeded initial state:
   in/<field>    every field before the first step,
   ent/<entry>/<field>  every field after each entry point of step 1,
   out<N>/<field> every field after N time steps (N = 1 and NSTEPS),
-plus the deck as JSON.  The reference itself is not run anywhere else, and
-never on the GPU box; these arrays are data only.
+plus the deck as JSON.  The reference itself is not run anywhere else (in
+particular not next to the GPU tests); these arrays are data only.
 
     python tests/golden/make_golden.py        # needs oracle/_ref built
 """

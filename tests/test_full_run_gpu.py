@@ -1,9 +1,9 @@
-"""Whole runs against the oracle on device 0: the configs' step counts on
+"""Whole runs against the oracle on device 0: the workloads' step counts on
 grids the oracle finishes in seconds.
 
-  * two-phase Buckley-Leverett at 40^3, 1000 steps (the config as specified)
+  * two-phase Buckley-Leverett at 40^3, 1000 steps (the workload as specified)
     and the three-phase injection physics at 48^3, 2000 steps (the 128^3
-    config's step count): every field bit-identical to the oracle after the
+    workload's step count): every field bit-identical to the oracle after the
     whole run (Corey closures);
   * the three-phase + energy model (reference deck, its closures or Corey-2;
     mu(T) and lambda_g(T) call exp/pow, the device library's, not glibc's)
