@@ -1,6 +1,6 @@
 /* The oracle: CPU restatement of capaz's explicit filtration step.
  *
- * TEST INFRASTRUCTURE ONLY -- the checker that the HIP backend is compared
+ * TEST INFRASTRUCTURE ONLY -- the reference that the HIP backend is compared
  * with.  Nothing in the product links or calls this file.
  *
  * Every expression below keeps the reference's IEEE operation order

@@ -2,7 +2,7 @@
  * step (cpu.cpp + Two-phase/two-phase.cpp + Three-phase/three-phase.cpp +
  * gauss.cpp + energy.cpp).  Test infrastructure only: tests, the
  * golden-vector script, smoke() and the CPU-baseline leg of bench.py call it
- * as the checker; the product never links it.
+ * to compare against; the product never links it.
  *
  * Layout: one double array per field, x fastest, local = i + j*Nx + k*Nx*Ny
  * (cpu.cpp:145).  Fields that a model does not have are NULL. */

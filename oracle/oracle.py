@@ -5,7 +5,7 @@
 keys (main.cpp:1327-1626) plus four keys of the backend's own: ``MODEL``
 (2 or 3), ``ENERGY`` (0/1), ``CLOSURE`` ("reference" or "corey") and
 ``COREY_N``.  Only tests, smoke() and the CPU-baseline leg of bench.py use
-this module, as the checker.
+this module, to compare against.
 """
 from __future__ import annotations
 

@@ -5,7 +5,7 @@ gauss.cpp, model .cpp and energy.cpp compiled by ``oracle/Makefile`` around
 ``ref_harness.cpp`` -- and exposes one single-domain reference context whose
 ``consts`` block is set from defines.ini keys (main.cpp:1327-1626) and whose
 host arrays are numpy views.  Only tests, the golden-vector script and the
-CPU-baseline leg of bench.py use this module, as the checker.
+CPU-baseline leg of bench.py use this module, to compare against.
 """
 from __future__ import annotations
 

@@ -4,7 +4,7 @@
 // shared_test.cpp and, for the energy build, energy.cpp) by oracle/Makefile
 // into oracle/_ref/libcapaz_ref_<model>.so.  Nothing in the product links it;
 // only tests, the golden-vector script and the CPU-baseline leg of bench.py
-// load it, as the checker.
+// load it, to compare against.
 //
 // The harness owns one single-domain reference context: a `consts` block set
 // from defines.ini keys (the key names and their meaning follow

@@ -5,7 +5,7 @@ backend's MODEL / ENERGY / CLOSURE / COREY_N keys.  ``REF_2PH`` and
 ``REF_3PH`` hold the values of the reference's own input decks
 (Two-phase/defines.ini, Three-phase/defines.ini); the grid keys are set per
 case.  States are drawn from numpy's PCG64 with an explicit seed so that the
-checker and the device see identical arrays.
+oracle and the device see identical arrays.
 """
 from __future__ import annotations
 
