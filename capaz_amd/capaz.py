@@ -254,15 +254,15 @@ class Domain:
 
     def tune(self, **kw) -> None:
         """Launch shape of the fused step (capaz_tune): fused_kernel, tile_y,
-        kchunk, adjoint_kernel, energy_kernel; timing=1 records events around
+        kchunk, adjoint_kernel, energy_kernel, energy_newton; timing=1 records events around
         the main kernel of each step."""
         for k, v in kw.items():
             self._check(load().capaz_tune(self._ctx, k.encode(), int(v)), f"capaz_tune({k}={v})")
 
     def tuning(self) -> dict:
         out = {}
-        for k in ("fused_kernel", "tile_y", "kchunk", "adjoint_kernel", "energy_kernel", "timing",
-                  "adjoint_segment"):
+        for k in ("fused_kernel", "tile_y", "kchunk", "adjoint_kernel", "energy_kernel", "energy_newton",
+                  "timing", "adjoint_segment"):
             v = ctypes.c_int()
             self._check(load().capaz_get_tuning(self._ctx, k.encode(), ctypes.byref(v)), "capaz_get_tuning")
             out[k] = v.value

@@ -328,7 +328,8 @@ int capaz_newton(capaz_ctx* c) {
     if ((rc = check_ctx(c))) return rc;
     if ((rc = materialize(c))) return rc;
     if (c->user.energy) {
-        CAPAZ_HIP(c, capaz::launch_newton_energy(c->p, fields_of(c), c->d_redo, c->stream));
+        CAPAZ_HIP(c, capaz::launch_newton_energy(c->p, fields_of(c), c->tuning.energy_newton ? c->d_redo : nullptr,
+                                                 c->stream));
         return CAPAZ_OK;
     }
     return run_entry(c, capaz::Entry::Newton, fields_of(c));
@@ -377,7 +378,8 @@ int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
         for (long s = 0; s < nsteps; ++s) {
             const KernelEvents* ev;
             if ((rc = next_events(c, &ev))) return rc;
-            CAPAZ_HIP(c, launch_energy_step(c->p, fields_of(c), c->d_lam, c->d_dtm, c->d_redo, c->stream, ev));
+            CAPAZ_HIP(c, launch_energy_step(c->p, fields_of(c), c->d_lam, c->d_dtm,
+                                            c->tuning.energy_newton ? c->d_redo : nullptr, c->stream, ev));
         }
         return CAPAZ_OK;
     }
@@ -426,6 +428,8 @@ int capaz_tune(capaz_ctx* c, const char* key, int value) {
         c->tuning.adjoint_kernel = value;
     else if (k == "energy_kernel" && (value == 0 || value == 1))
         c->tuning.energy_kernel = value;
+    else if (k == "energy_newton" && (value == 0 || value == 1))
+        c->tuning.energy_newton = value;
     else if (k == "timing" && (value == 0 || value == 1)) {
         int rc;
         if ((rc = check_ctx(c)) || (rc = drain_events(c))) return rc;
@@ -445,6 +449,7 @@ int capaz_get_tuning(const capaz_ctx* c, const char* key, int* value) {
     else if (k == "kchunk") *value = c->tuning.kchunk;
     else if (k == "adjoint_kernel") *value = c->tuning.adjoint_kernel;
     else if (k == "energy_kernel") *value = c->tuning.energy_kernel;
+    else if (k == "energy_newton") *value = c->tuning.energy_newton;
     else if (k == "timing") *value = c->timing ? 1 : 0;
     else if (k == "adjoint_segment") *value = (int)std::min<long>(c->adj_segment, 0x7fffffff);
     else return CAPAZ_ERR_INVALID;

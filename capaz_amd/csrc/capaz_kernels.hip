@@ -325,6 +325,7 @@ hipError_t launch_energy_e_tiles(const DevParams& p, const DevFields& F, const d
 }
 
 hipError_t launch_newton_energy(const DevParams& p, const DevFields& F, unsigned char* redo, hipStream_t s) {
+    if (!redo) return launch_entry(Entry::Newton, p, F, s);
     hipLaunchKernelGGL(k_newton_energy, grid_for(p.n), dim3(kBlock), 0, s, p, F, redo);
     hipLaunchKernelGGL(k_newton_energy_redo, grid_for((p.n + 15) / 16), dim3(kBlock), 0, s, p, F, redo);
     return hipGetLastError();

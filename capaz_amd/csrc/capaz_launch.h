@@ -22,12 +22,16 @@ enum class Entry { S, P_Xi_ro, U, RoS, HE, E, Newton, Boundary };
 // and lambda_eff per cell; boundary u; roS and interior u on z tiles; E on z
 // tiles from the stored lambda_eff; Newton; boundary faces), 0 = the eight
 // entry-point kernels.
+// energy_newton: 1 = the energy model's Newton in two passes (every interior
+// cell without the pivot search, then the cells that need it), 0 = the
+// general routine in one pass; the results are the same.
 struct Tuning {
     int fused_kernel = 1;
     int tile_y = 8;
     int kchunk = 0;
     int adjoint_kernel = 2;
     int energy_kernel = 1;
+    int energy_newton = 1;
 };
 
 // Optional events recorded around the fused step's main kernel (the
@@ -50,12 +54,14 @@ hipError_t launch_roS_stored(const DevParams& p, const DevFields& F, const doubl
 hipError_t launch_energy_e_tiles(const DevParams& p, const DevFields& F, const double* lam, hipStream_t s);
 // One step of the energy model in six passes, in place on F; lam is a work
 // array of Nx*Ny*Nz doubles, dtm holds dt / m per cell, redo Nx*Ny*Nz bytes
-// (rounded up to 16) for the Newton pass; ev brackets the whole step.
+// (rounded up to 16) for the Newton pass (NULL: the Newton in one pass, as
+// launch_newton_energy); ev brackets the whole step.
 hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* lam, const double* dtm,
                               unsigned char* redo, hipStream_t s, const KernelEvents* ev = nullptr);
 // The energy model's Newton entry point: interior cells without the pivot
 // search, then the cells that need it (marked in redo, Nx*Ny*Nz bytes rounded
-// up to 16) with the general routine.
+// up to 16) with the general routine; redo NULL: the general routine on every
+// cell in one pass.
 hipError_t launch_newton_energy(const DevParams& p, const DevFields& F, unsigned char* redo, hipStream_t s);
 // Planes per tile of the fused z kernels: kchunk if >= 1, else chosen from
 // the grid and the number of workgroups resident at once, so that small grids

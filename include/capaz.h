@@ -172,6 +172,10 @@ int capaz_step(capaz_ctx* ctx, long first_step, long nsteps);
  *                   cell S, P_Xi_ro, HE and the conductivity; u; roS; E
  *                   from the stored conductivity; Newton; the boundary
  *                   faces); 0 the eight entry-point kernels
+ *   "energy_newton" the energy model's Newton (its entry point and step):
+ *                   1 (default) every interior cell without the pivot
+ *                   search, then the cells that need it with the search;
+ *                   0 the general routine on every cell in one pass
  *   "timing"        1: record HIP events around each fused step's main
  *                   kernel (the energy model's whole step; read with
  *                   capaz_kernel_time); 0: off (default).

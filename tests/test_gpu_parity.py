@@ -178,6 +178,33 @@ def test_energy_pivot_search_cells_vs_oracle():
     d.close()
 
 
+@pytest.mark.parametrize("gas_free", [False, True])
+def test_energy_newton_two_passes_equal_the_general_routine(gas_free):
+    """On the device: the energy model's Newton in two passes (energy_newton
+    1: no pivot search, then the refused cells with it) and the general
+    routine in one pass (energy_newton 0) give every field the same bits,
+    through the six-pass step and through the entry-point sequence; the
+    gas-free case (incompressible water and oil, S_w + S_n = 1 in four cells)
+    sends cells to the second pass."""
+    keys, st = workloads.energy_3ph(20, 18, 14)
+    if gas_free:
+        keys = dict(keys, BETA_W=0.0, BETA_N=0.0)
+        st = {f: np.array(v, copy=True) for f, v in st.items()}
+        for c in ((3, 4, 5), (4, 5, 6), (5, 2, 8), (10, 10, 10)):
+            st["S_w"][c], st["S_n"][c] = 0.625, 0.375
+    for energy_kernel in (1, 0):
+        got = []
+        for newton in (1, 0):
+            d = capaz.Domain(keys, st)
+            d.tune(energy_kernel=energy_kernel, energy_newton=newton)
+            assert d.tuning()["energy_newton"] == newton
+            d.run(3)
+            got.append(d.fields())
+            d.close()
+        for f, v in got[1].items():
+            assert bit_equal(got[0][f], v), (energy_kernel, f)
+
+
 def test_full_size_fused_equals_sequence():
     """256^3 headline workload: two fused steps and two entry-point sequences
     give identical bits in every field."""
