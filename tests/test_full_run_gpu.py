@@ -14,6 +14,9 @@ grids the oracle finishes in seconds.
     properties that hold at any size: batched = single-domain bits per
     realization; gradient . v = tangent derivative along v to rounding.
 """
+import os
+import sys
+
 import numpy as np
 import pytest
 
@@ -109,4 +112,26 @@ def test_inversion_full_size_gradient_matches_the_tangent():
         ad = float(np.sum(gK * vK) + np.sum(gm * vm))
         assert Jt == J
         assert abs(dJ - ad) <= 1e-12 * abs(dJ), (dJ, ad)
+    d.close()
+
+
+@pytest.mark.skipif(os.environ.get("CAPAZ_FULL_RUN") != "1",
+                    reason="~6 minutes of oracle time on 16 host threads; set CAPAZ_FULL_RUN=1")
+def test_headline_full_run_is_bit_identical():
+    """The benchmark workload itself: three-phase 256^3, 1000 steps (Corey-2,
+    lognormal K, top injection), every primary field bit-identical to the
+    oracle after every 100 steps and every field after the last."""
+    keys, st = workloads.headline(256)
+    od = oracle.OracleDomain(keys, st)
+    d = capaz.Domain(keys, st)
+    del st
+    for done in range(0, 1000, 100):
+        od.run(100, first_step=done + 1)
+        d.run(100, first_step=done + 1)
+        for f in ("P_w", "S_w", "S_n"):
+            assert bit_equal(d.field(f), od.field(f)), (done + 100, f, max_rel(d.field(f), od.field(f)))
+        print(f"headline full run: {done + 100} steps bit-identical", file=sys.stderr, flush=True)
+    want = od.fields()
+    for f, v in want.items():
+        assert bit_equal(d.field(f), v), (f, max_rel(d.field(f), v))
     d.close()
