@@ -52,6 +52,7 @@ EXPORTS = {
     "capaz_tune": (ctypes.c_int, ["ctx", ctypes.c_char_p, ctypes.c_int]),
     "capaz_get_tuning": (ctypes.c_int, ["ctx", ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
     "capaz_kernel_time": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]),
+    "capaz_check_finite": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_int64)]),
     "capaz_sync": (ctypes.c_int, ["ctx"]),
     "capaz_stream": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_void_p)]),
     "capaz_load_face": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
@@ -108,7 +109,22 @@ _lib = None
 
 
 class CapazError(RuntimeError):
-    pass
+    """A C ABI call failed; ``code`` is its capaz_status."""
+
+    def __init__(self, msg: str, code: int = 0):
+        super().__init__(msg)
+        self.code = code
+
+
+class NonFiniteState(CapazError):
+    """CAPAZ_ERR_NONFINITE: the state holds a NaN or an infinity at cell ``first_bad``."""
+
+    def __init__(self, msg: str, first_bad: int):
+        super().__init__(msg, CAPAZ_ERR_NONFINITE)
+        self.first_bad = first_bad
+
+
+CAPAZ_ERR_NONFINITE = -5
 
 
 def library_path() -> str:
@@ -185,7 +201,10 @@ class Domain:
     def _check(self, rc: int, what: str) -> None:
         if rc != 0:
             msg = load().capaz_last_error(self._ctx).decode()
-            raise CapazError(f"{what} failed ({rc}): {msg}")
+            if rc == CAPAZ_ERR_NONFINITE:
+                cell = int(msg.split("cell ", 1)[1].split(" ", 1)[0]) if "cell " in msg else -1
+                raise NonFiniteState(f"{what} failed ({rc}): {msg}", cell)
+            raise CapazError(f"{what} failed ({rc}): {msg}", rc)
 
     # -- data (load_data_to_device / load_data_to_host) ---------------------
     def set_field(self, name: str, value) -> None:
@@ -255,14 +274,15 @@ class Domain:
     def tune(self, **kw) -> None:
         """Launch shape of the fused step (capaz_tune): fused_kernel, tile_y,
         kchunk, adjoint_kernel, energy_kernel, energy_newton; timing=1 records events around
-        the main kernel of each step."""
+        the main kernel of each step; check_every=K runs the non-finite
+        sentinel after every K-th step."""
         for k, v in kw.items():
             self._check(load().capaz_tune(self._ctx, k.encode(), int(v)), f"capaz_tune({k}={v})")
 
     def tuning(self) -> dict:
         out = {}
-        for k in ("fused_kernel", "tile_y", "kchunk", "adjoint_kernel", "energy_kernel", "energy_newton",
-                  "timing", "adjoint_segment"):
+        for k in ("check_every", "fused_kernel", "tile_y", "kchunk", "adjoint_kernel", "energy_kernel",
+                  "energy_newton", "timing", "adjoint_segment"):
             v = ctypes.c_int()
             self._check(load().capaz_get_tuning(self._ctx, k.encode(), ctypes.byref(v)), "capaz_get_tuning")
             out[k] = v.value
@@ -316,6 +336,13 @@ class Domain:
         ms, n = ctypes.c_double(), ctypes.c_long()
         self._check(load().capaz_kernel_time(self._ctx, ctypes.byref(ms), ctypes.byref(n)), "capaz_kernel_time")
         return ms.value, n.value
+
+    def check_finite(self) -> int:
+        """capaz_check_finite: -1 if the primary state is finite, else raises
+        NonFiniteState with the first bad cell."""
+        first = ctypes.c_int64(-1)
+        self._check(load().capaz_check_finite(self._ctx, ctypes.byref(first)), "capaz_check_finite")
+        return first.value
 
     def sync(self) -> None:
         self._check(load().capaz_sync(self._ctx), "capaz_sync")

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -152,6 +153,52 @@ int next_events(capaz_ctx* c, const capaz::KernelEvents** out) {
     return CAPAZ_OK;
 }
 
+// The non-finite sentinel over the primary state (P_w, S_n; S_w for three
+// phases; T with the energy equation): *first = the smallest cell index
+// holding a NaN or an infinity in one of them, -1 if none.
+int find_nonfinite(capaz_ctx* c, int64_t* first) {
+    using namespace capaz;
+    const double* f[4] = {c->cur[F_P_w], c->cur[F_S_n], c->user.model == 3 ? c->cur[F_S_w] : nullptr,
+                          c->user.energy ? c->cur[F_T] : nullptr};
+    CAPAZ_HIP(c, launch_find_nonfinite(c->p, f, c->d_first, c->stream));
+    unsigned long long h = 0;
+    CAPAZ_HIP(c, hipMemcpyAsync(&h, c->d_first, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    CAPAZ_HIP(c, hipStreamSynchronize(c->stream));
+    *first = h == ~0ull ? -1 : (int64_t)h;
+    return CAPAZ_OK;
+}
+
+// CAPAZ_ERR_NONFINITE with a message naming cell l, its (i, j, k) and the
+// state fields that are not finite there (after step `step`, if >= 0).
+int nonfinite_error(capaz_ctx* c, int64_t l, long step) {
+    using namespace capaz;
+    const int ids[4] = {F_P_w, F_S_n, F_S_w, F_T};
+    const char* names[4] = {"P_w", "S_n", "S_w", "T"};
+    std::string which;
+    for (int q = 0; q < 4; ++q) {
+        if ((q == 2 && c->user.model != 3) || (q == 3 && !c->user.energy)) continue;
+        double v = 0.;
+        CAPAZ_HIP(c, hipMemcpyAsync(&v, c->cur[ids[q]] + l, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+        CAPAZ_HIP(c, hipStreamSynchronize(c->stream));
+        if (!std::isfinite(v)) which += std::string(which.empty() ? "" : ", ") + names[q] + " = " + std::to_string(v);
+    }
+    const int64_t i = l % c->p.Nx, j = (l / c->p.Nx) % c->p.Ny, k = l / ((int64_t)c->p.Nx * c->p.Ny);
+    std::string msg = "non-finite state at cell " + std::to_string(l) + " (i " + std::to_string(i) + ", j " +
+                      std::to_string(j) + ", k " + std::to_string(k) + "): " + which;
+    if (step >= 0) msg += " after step " + std::to_string(step);
+    return fail(c, CAPAZ_ERR_NONFINITE, msg);
+}
+
+// The sentinel of capaz_step after step number `step` (every check_every-th).
+int check_after_step(capaz_ctx* c, long step) {
+    const int every = c->tuning.check_every;
+    if (every <= 0 || step % every != 0) return CAPAZ_OK;
+    int64_t first = -1;
+    int rc;
+    if ((rc = find_nonfinite(c, &first))) return rc;
+    return first < 0 ? CAPAZ_OK : nonfinite_error(c, first, step);
+}
+
 int scan_m(capaz_ctx* c) {
     if (c->m_checked) return CAPAZ_OK;
     CAPAZ_HIP(c, hipMemsetAsync(c->d_flag, 0, sizeof(int), c->stream));
@@ -207,6 +254,7 @@ int capaz_create(const capaz_params* u, capaz_ctx** out) {
     if ((e = hipSetDevice(u->device)) != hipSuccess) return bail(e, "hipSetDevice");
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return bail(e, "hipStreamCreate");
     if ((e = hipMalloc(&c->d_flag, sizeof(int))) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = hipMalloc(&c->d_first, sizeof(unsigned long long))) != hipSuccess) return bail(e, "hipMalloc");
     const size_t bytes = (size_t)c->p.n * sizeof(double);
     if ((e = hipMalloc(&c->d_dtm, bytes)) != hipSuccess) return bail(e, "hipMalloc");
     if (u->energy && (e = hipMalloc(&c->d_lam, bytes)) != hipSuccess) return bail(e, "hipMalloc");
@@ -236,6 +284,7 @@ int capaz_destroy(capaz_ctx* c) {
         if (c->alt[f]) (void)hipFree(c->alt[f]);
     }
     if (c->d_flag) (void)hipFree(c->d_flag);
+    if (c->d_first) (void)hipFree(c->d_first);
     if (c->d_dtm) (void)hipFree(c->d_dtm);
     if (c->d_lam) (void)hipFree(c->d_lam);
     if (c->d_redo) (void)hipFree(c->d_redo);
@@ -380,14 +429,17 @@ int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
             if ((rc = next_events(c, &ev))) return rc;
             CAPAZ_HIP(c, launch_energy_step(c->p, fields_of(c), c->d_lam, c->d_dtm,
                                             c->tuning.energy_newton ? c->d_redo : nullptr, c->stream, ev));
+            if ((rc = check_after_step(c, first_step + s))) return rc;
         }
         return CAPAZ_OK;
     }
     if (c->user.energy || c->m_has_zero) {
         // Interior cells with m == 0 keep the reference's stale-roS rule,
         // which only the sequence reproduces.
-        for (long s = first_step; s < first_step + nsteps; ++s)
+        for (long s = first_step; s < first_step + nsteps; ++s) {
             if ((rc = capaz_step_sequence(c, (double)s * c->user.dt))) return rc;
+            if ((rc = check_after_step(c, s))) return rc;
+        }
         return CAPAZ_OK;
     }
     for (long s = 0; s < nsteps; ++s) {
@@ -411,6 +463,7 @@ int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
         for (int f = 0; f < F_COUNT; ++f)
             if (c->primary[f]) std::swap(c->cur[f], c->alt[f]);
         c->stale = true;
+        if ((rc = check_after_step(c, first_step + s))) return rc;
     }
     return CAPAZ_OK;
 }
@@ -418,7 +471,9 @@ int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
 int capaz_tune(capaz_ctx* c, const char* key, int value) {
     if (!c || !key) return CAPAZ_ERR_INVALID;
     const std::string k(key);
-    if (k == "fused_kernel" && (value >= 0 && value <= 2))
+    if (k == "check_every" && value >= 0)
+        c->tuning.check_every = value;
+    else if (k == "fused_kernel" && (value >= 0 && value <= 2))
         c->tuning.fused_kernel = value;
     else if (k == "tile_y" && (value == 4 || value == 8))
         c->tuning.tile_y = value;
@@ -444,7 +499,8 @@ int capaz_tune(capaz_ctx* c, const char* key, int value) {
 int capaz_get_tuning(const capaz_ctx* c, const char* key, int* value) {
     if (!c || !key || !value) return CAPAZ_ERR_INVALID;
     const std::string k(key);
-    if (k == "fused_kernel") *value = c->tuning.fused_kernel;
+    if (k == "check_every") *value = c->tuning.check_every;
+    else if (k == "fused_kernel") *value = c->tuning.fused_kernel;
     else if (k == "tile_y") *value = c->tuning.tile_y;
     else if (k == "kchunk") *value = c->tuning.kchunk;
     else if (k == "adjoint_kernel") *value = c->tuning.adjoint_kernel;
@@ -464,6 +520,15 @@ int capaz_kernel_time(capaz_ctx* c, double* total_ms, long* launches) {
     *total_ms = c->timed_ms;
     *launches = c->timed_launches;
     return CAPAZ_OK;
+}
+
+int capaz_check_finite(capaz_ctx* c, int64_t* first_bad) {
+    int rc;
+    if ((rc = check_ctx(c))) return rc;
+    if (!first_bad) return fail(c, CAPAZ_ERR_INVALID, "NULL first_bad");
+    *first_bad = -1;
+    if ((rc = find_nonfinite(c, first_bad))) return rc;
+    return *first_bad < 0 ? CAPAZ_OK : nonfinite_error(c, *first_bad, -1);
 }
 
 int capaz_sync(capaz_ctx* c) {

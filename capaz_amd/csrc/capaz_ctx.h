@@ -21,6 +21,7 @@ struct capaz_ctx {
     bool m_checked = false; // m has been scanned for zeros since it last changed
     bool m_has_zero = false;
     int* d_flag = nullptr;
+    unsigned long long* d_first = nullptr;  // non-finite sentinel: first bad cell
     double* d_dtm = nullptr;  // dt / m per cell, refreshed with the porosity scan
     double* d_lam = nullptr;  // energy model: lambda_eff per cell within a step
     unsigned char* d_redo = nullptr;  // energy model: cells the Newton pass redoes with pivoting

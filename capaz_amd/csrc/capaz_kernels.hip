@@ -140,6 +140,31 @@ __global__ void __launch_bounds__(kBlock) k_face(DevParams p, double* field, dou
         buf[t] = field[l];
 }
 
+// Non-finite sentinel: the smallest cell index at which one of up to four
+// state fields holds a NaN or an infinity, by atomicMin into *first (set to
+// ~0 beforehand by k_fill_u64, a kernel rather than a memset so that the
+// pair may be captured in a graph).  One atomic per wavefront that sees a
+// bad cell.
+__global__ void __launch_bounds__(kBlock) k_fill_u64(unsigned long long* dst, unsigned long long v) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dst = v;
+}
+__global__ void __launch_bounds__(kBlock) k_nonfinite(DevParams p, const double* f0, const double* f1,
+                                                      const double* f2, const double* f3,
+                                                      unsigned long long* first) {
+    const int64_t l = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    bool bad = false;
+    if (l < p.n) {
+        bad = !isfinite(f0[l]) || !isfinite(f1[l]);
+        if (f2) bad = bad || !isfinite(f2[l]);
+        if (f3) bad = bad || !isfinite(f3[l]);
+    }
+    const unsigned long long mask = __ballot(bad);
+    if (mask == 0ull) return;
+    // the lowest bad lane of the wavefront holds its smallest index
+    const int lane = (int)(threadIdx.x & 63u);
+    if (lane == __ffsll((long long)mask) - 1) atomicMin(first, (unsigned long long)l);
+}
+
 inline dim3 grid_for(int64_t n) { return dim3((unsigned)((n + kBlock - 1) / kBlock)); }
 
 }  // namespace
@@ -364,6 +389,13 @@ hipError_t launch_copy_interior(const DevParams& p, const double* src, double* d
 
 hipError_t launch_m_zero(const DevParams& p, const double* m, int* flag, hipStream_t s) {
     hipLaunchKernelGGL(k_m_zero, grid_for(p.n), dim3(kBlock), 0, s, p, m, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_find_nonfinite(const DevParams& p, const double* const* f, unsigned long long* first,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kBlock), 0, s, first, ~0ull);
+    hipLaunchKernelGGL(k_nonfinite, grid_for(p.n), dim3(kBlock), 0, s, p, f[0], f[1], f[2], f[3], first);
     return hipGetLastError();
 }
 

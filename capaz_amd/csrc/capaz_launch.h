@@ -26,6 +26,7 @@ enum class Entry { S, P_Xi_ro, U, RoS, HE, E, Newton, Boundary };
 // cell without the pivot search, then the cells that need it), 0 = the
 // general routine in one pass; the results are the same.
 struct Tuning {
+    int check_every = 0;  // non-finite sentinel after every check_every-th step (0: off)
     int fused_kernel = 1;
     int tile_y = 8;
     int kchunk = 0;
@@ -75,5 +76,9 @@ hipError_t launch_dtm(const DevParams& p, const double* m, double* dtm, hipStrea
 hipError_t launch_copy_interior(const DevParams& p, const double* src, double* dst, hipStream_t s);
 hipError_t launch_m_zero(const DevParams& p, const double* m, int* flag, hipStream_t s);
 hipError_t launch_face(const DevParams& p, double* field, double* buf, int axis, int pos, int save, hipStream_t s);
+// *first = the smallest cell index at which f[0], f[1] or (where not NULL)
+// f[2], f[3] is not finite, ~0 if there is none.
+hipError_t launch_find_nonfinite(const DevParams& p, const double* const* f, unsigned long long* first,
+                                 hipStream_t s);
 
 }  // namespace capaz

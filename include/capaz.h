@@ -41,7 +41,8 @@ typedef enum capaz_status {
     CAPAZ_ERR_INVALID = -1, /* bad argument or unsupported configuration */
     CAPAZ_ERR_DEVICE = -2,  /* a HIP call failed */
     CAPAZ_ERR_NOMEM = -3,   /* device allocation failed */
-    CAPAZ_ERR_STATE = -4    /* call not valid in the context's current state */
+    CAPAZ_ERR_STATE = -4,   /* call not valid in the context's current state */
+    CAPAZ_ERR_NONFINITE = -5 /* the state holds a NaN or an infinity (capaz_check_finite) */
 } capaz_status;
 
 typedef enum capaz_model {
@@ -182,6 +183,11 @@ int capaz_step(capaz_ctx* ctx, long first_step, long nsteps);
  *   "adjoint_segment" (read only) the segment length of the last
  *                   capaz_adjoint
  *                   Setting it resets the totals.
+ *   "check_every"   K > 0: capaz_step runs capaz_check_finite after every
+ *                   step whose number is a multiple of K and stops with
+ *                   CAPAZ_ERR_NONFINITE at the first failing check (the
+ *                   context then holds the state after that step); 0: off
+ *                   (default).
  * Returns CAPAZ_ERR_INVALID for an unknown key or value.  No counterpart in
  * the reference (gpu.cu's launch shape is fixed). */
 int capaz_tune(capaz_ctx* ctx, const char* key, int value);
@@ -191,6 +197,15 @@ int capaz_get_tuning(const capaz_ctx* ctx, const char* key, int* value);
  * timed since "timing" was last set; waits for queued work.  No counterpart
  * in the reference. */
 int capaz_kernel_time(capaz_ctx* ctx, double* total_ms, long* launches);
+
+/* Non-finite sentinel over the primary state (P_w, S_n; S_w for three
+ * phases; T with the energy equation), one pass over those fields on the
+ * device.  CAPAZ_OK and *first_bad = -1 when every value is finite; else
+ * CAPAZ_ERR_NONFINITE, *first_bad = the smallest linear index
+ * i + j*Nx + k*Nx*Ny of a cell holding a NaN or an infinity, and
+ * capaz_last_error names the cell and the fields.  Replaces the printed-only
+ * checks of test_nan (shared_test.cpp:56-70), which never stop a run. */
+int capaz_check_finite(capaz_ctx* ctx, int64_t* first_bad);
 
 /* Waits for all work queued on the context. */
 int capaz_sync(capaz_ctx* ctx);

@@ -120,3 +120,32 @@ def bit_equal(a: np.ndarray, b: np.ndarray) -> bool:
     a = np.asarray(a)
     b = np.asarray(b)
     return bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
+
+
+# Input decks run end to end through the reference driver's loop (main.cpp:7-68):
+# the plot files the reference CPU program writes for them are committed
+# fixtures (tests/golden/plots_<id>.npz, written by tests/golden/make_golden.py).
+PLOT_DECKS = [
+    ("2ph", case("2ph", 15, 15, 1, TIMEX=0.2, SAVE_PLOTS=100, PRINT_SCREEN=100)),
+    ("2ph", case("2ph", 6, 7, 5, TIMEX=0.02, SAVE_PLOTS=7, PRINT_SCREEN=5, SOURCE=1)),
+    ("3ph", case("3ph", 1, 15, 1, TIMEX=0.2, SAVE_PLOTS=100, PRINT_SCREEN=100)),
+    ("3ph", case("3ph", 10, 12, 9, DT=1e-4, TIMEX=0.005, SAVE_PLOTS=25, PRINT_SCREEN=25, SOURCE=1)),
+    ("3ph", case("3ph", 9, 8, 1, DT=1e-4, TIMEX=0.003, SAVE_PLOTS=10, PRINT_SCREEN=10)),
+    ("3phe", case("3phe", 1, 9, 1, TIMEX=0.02, SAVE_PLOTS=10, PRINT_SCREEN=10)),
+    ("3phe", case("3phe", 5, 6, 4, DT=1e-4, TIMEX=0.002, SAVE_PLOTS=10, PRINT_SCREEN=10)),
+]
+
+
+def plot_deck_id(model: str, keys: dict) -> str:
+    return f"{model}-{keys['NX']}x{keys['NY']}x{keys['NZ']}"
+
+
+def write_deck(keys: dict, workdir: str) -> None:
+    """defines.ini of `keys` (the reference's keys only) in workdir."""
+    import os
+
+    os.makedirs(workdir, exist_ok=True)
+    with open(os.path.join(workdir, "defines.ini"), "w") as f:
+        f.write("# deck\n#\n")
+        for k, v in ref_keys(keys).items():
+            f.write(f"{k}={v}\n")

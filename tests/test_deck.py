@@ -5,57 +5,33 @@ driver (capaz_amd.deck).
     lines, key up to '=', value up to the first space, atof/atoi prefixes,
     model-specific keys;
   * run on the oracle, the deck runner writes plot files byte-identical to
-    the ones the reference CPU program (oracle/_ref/capaz_cpu_<model>, built
-    from the reference's own main.cpp by `make -C oracle dropin`) writes for
-    the same deck: reader, initial state, loop, file names, headers, row
-    order and number formats all agree (skipped where the program is
-    absent);
+    the ones the reference CPU program (built from the reference's own
+    main.cpp) writes for the same deck -- committed as fixtures
+    tests/golden/plots_<deck>.npz by tests/golden/make_golden.py: reader,
+    initial state, loop, file names, headers, row order and number formats
+    all agree;
   * on the GPU the same decks give the reference program's plots to the
     printed precision, and the fused mode writes the same files as the
     entry-point sequence.
 """
 import math
 import os
-import subprocess
 
 import numpy as np
 import pytest
 
 from capaz_amd import deck, io
-from capaz_cases import REF_2PH, REF_3PH, case, ref_keys
+from capaz_cases import PLOT_DECKS, REF_2PH, REF_3PH, case, plot_deck_id, ref_keys, write_deck
+from golden.make_golden import load_plots
 
-REF = os.path.join(os.path.dirname(os.path.dirname(__file__)), "oracle", "_ref")
-
-DECKS = [
-    ("2ph", case("2ph", 15, 15, 1, TIMEX=0.2, SAVE_PLOTS=100, PRINT_SCREEN=100)),
-    ("2ph", case("2ph", 6, 7, 5, TIMEX=0.02, SAVE_PLOTS=7, PRINT_SCREEN=5, SOURCE=1)),
-    ("3ph", case("3ph", 1, 15, 1, TIMEX=0.2, SAVE_PLOTS=100, PRINT_SCREEN=100)),
-    ("3ph", case("3ph", 10, 12, 9, DT=1e-4, TIMEX=0.005, SAVE_PLOTS=25, PRINT_SCREEN=25, SOURCE=1)),
-    ("3ph", case("3ph", 9, 8, 1, DT=1e-4, TIMEX=0.003, SAVE_PLOTS=10, PRINT_SCREEN=10)),
-    ("3phe", case("3phe", 1, 9, 1, TIMEX=0.02, SAVE_PLOTS=10, PRINT_SCREEN=10)),
-    ("3phe", case("3phe", 5, 6, 4, DT=1e-4, TIMEX=0.002, SAVE_PLOTS=10, PRINT_SCREEN=10)),
-]
-IDS = [f"{m}-{d['NX']}x{d['NY']}x{d['NZ']}" for m, d in DECKS]
+DECKS = PLOT_DECKS
+IDS = [plot_deck_id(m, d) for m, d in DECKS]
 MODEL = {"2ph": (2, 0), "3ph": (3, 0), "3phe": (3, 1)}
 
 
-def write_deck(keys, workdir):
-    os.makedirs(workdir, exist_ok=True)
-    with open(os.path.join(workdir, "defines.ini"), "w") as f:
-        f.write("# deck\n#\n")
-        for k, v in ref_keys(keys).items():
-            f.write(f"{k}={v}\n")
-
-
-def reference_plots(model, keys, workdir):
-    binary = os.path.join(REF, f"capaz_cpu_{model}")
-    if not os.path.exists(binary):
-        pytest.skip("reference CPU program not built (make -C oracle dropin needs the reference sources)")
-    write_deck(keys, workdir)
-    r = subprocess.run([binary], cwd=workdir, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
-                       timeout=120)
-    assert r.returncode == 0, r.stdout[-2000:]
-    return read_plots(workdir), r.stdout
+def reference_plots(model, keys):
+    """The reference CPU program's plot files and printed text for a deck."""
+    return load_plots(model, keys)
 
 
 def read_plots(workdir):
@@ -212,7 +188,7 @@ def test_c_row_writer_matches_python_formatting_and_checks_formats(tmp_path):
 def test_deck_on_the_oracle_writes_the_reference_programs_plots(model, keys, tmp_path):
     from oracle import oracle
 
-    want, out = reference_plots(model, keys, str(tmp_path / "ref"))
+    want, out = reference_plots(model, keys)
     write_deck(keys, str(tmp_path / "ours"))
     lines = []
     m, e = MODEL[model]
@@ -238,7 +214,7 @@ def numbers(text):
 @pytest.mark.gpu
 @pytest.mark.parametrize("model,keys", DECKS, ids=IDS)
 def test_deck_on_the_gpu_matches_the_reference_program(model, keys, tmp_path):
-    want, _ = reference_plots(model, keys, str(tmp_path / "ref"))
+    want, _ = reference_plots(model, keys)
     m, e = MODEL[model]
     write_deck(keys, str(tmp_path / "seq"))
     r = deck.run_deck(str(tmp_path / "seq"), m, e, log=silent)

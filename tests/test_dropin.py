@@ -1,11 +1,10 @@
 """Drop-in integration: the reference's own driver (main.cpp, the model .cpp,
 gauss.cpp, no_communication.cpp, shared_test.cpp) linked against the
 capaz_amd adapter and libcapaz_amd.so in place of cpu.cpp
-(oracle/_ref/capaz_dropin_<m>), run on an input deck, writes the same Tecplot
-plot files as the reference CPU program built from the same sources
-(oracle/_ref/capaz_cpu_<m>).  Both binaries are built in this container by
-`make -C oracle dropin` (they need the reference sources); the test is
-skipped where they are absent.
+(oracle/_ref/capaz_dropin_<m>, built by `make -C oracle dropin`, which
+__graft_entry__.build() runs where the reference sources are present), run on
+an input deck, writes the Tecplot plot files the reference CPU program writes
+for it (committed fixtures tests/golden/plots_<deck>.npz).
 
 Plots print six significant digits (main.cpp:700-780); every column is
 compared relative to its largest magnitude, since near-zero velocities are
@@ -16,25 +15,18 @@ import subprocess
 import numpy as np
 import pytest
 
-from capaz_cases import case, ref_keys
+from capaz_cases import PLOT_DECKS, plot_deck_id, write_deck
+from golden.make_golden import load_plots
 
 pytestmark = pytest.mark.gpu
 
 REF = os.path.join(os.path.dirname(os.path.dirname(__file__)), "oracle", "_ref")
 
-DECKS = [
-    ("2ph", case("2ph", 15, 15, 1, TIMEX=0.2, SAVE_PLOTS=100, PRINT_SCREEN=100)),
-    ("3ph", case("3ph", 1, 15, 1, TIMEX=0.2, SAVE_PLOTS=100, PRINT_SCREEN=100)),
-    ("3ph", case("3ph", 10, 12, 9, DT=1e-4, TIMEX=0.005, SAVE_PLOTS=25, PRINT_SCREEN=25, SOURCE=1)),
-    ("3phe", case("3phe", 1, 9, 1, TIMEX=0.02, SAVE_PLOTS=10, PRINT_SCREEN=10)),
-]
+DECKS = [PLOT_DECKS[i] for i in (0, 2, 3, 5)]
 
 
 def run(binary, deck, workdir):
-    os.makedirs(workdir, exist_ok=True)
-    with open(os.path.join(workdir, "defines.ini"), "w") as f:
-        for k, v in ref_keys(deck).items():
-            f.write(f"{k}={v}\n")
+    write_deck(deck, workdir)
     r = subprocess.run([binary], cwd=workdir, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
                        timeout=120)
     assert r.returncode == 0, r.stdout[-2000:]
@@ -47,12 +39,11 @@ def numbers(text):
     return np.array([[float(x) for x in r] for r in rows])
 
 
-@pytest.mark.parametrize("model,deck", DECKS, ids=[f"{m}-{d['NX']}x{d['NY']}x{d['NZ']}" for m, d in DECKS])
+@pytest.mark.parametrize("model,deck", DECKS, ids=[plot_deck_id(m, d) for m, d in DECKS])
 def test_dropin_driver_reproduces_reference_plots(model, deck, tmp_path):
-    cpu, gpu = os.path.join(REF, f"capaz_cpu_{model}"), os.path.join(REF, f"capaz_dropin_{model}")
-    if not (os.path.exists(cpu) and os.path.exists(gpu)):
-        pytest.skip("drop-in drivers not built (make -C oracle dropin needs the reference sources)")
-    a = run(cpu, deck, str(tmp_path / "cpu"))
+    gpu = os.path.join(REF, f"capaz_dropin_{model}")
+    assert os.path.exists(gpu), "drop-in driver not built: __graft_entry__.build() builds it (make -C oracle dropin)"
+    a, _ = load_plots(model, deck)
     b = run(gpu, deck, str(tmp_path / "gpu"))
     assert list(a) == list(b) and len(a) >= 2
     for name in a:

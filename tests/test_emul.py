@@ -107,3 +107,25 @@ def test_energy_newton_two_passes_equal_the_general_routine(closure, gas_free):
     assert redone >= 3 if gas_free else redone == 0
     for f, v in a.fields().items():
         assert bit_equal(b.field(f), v), f
+
+
+def test_pivot_search_state_host_build_matches_oracle():
+    """The kernels' bodies built for the host on the state whose Newton
+    matrices all need the signed pivot search (interior S_w in
+    [0.005, 0.02] at beta_w 4.5e-10): three entry-point steps and three
+    fused steps equal the oracle bit for bit, every primary field."""
+    from test_gpu_parity import pivot_search_state
+
+    keys, st = pivot_search_state(12)
+    od = oracle.OracleDomain(keys, st)
+    od.run(3)
+    seq = emul.EmulDomain(keys, st)
+    seq.run(3)
+    fused = emul.EmulDomain(keys, st)
+    for _ in range(3):
+        fused.fused_step()
+    for f in ("P_w", "S_w", "S_n"):
+        assert bit_equal(seq.field(f), od.field(f)), f
+        assert bit_equal(fused.field(f), od.field(f)), f
+    for f in od.names:
+        assert bit_equal(seq.field(f), od.field(f)), f

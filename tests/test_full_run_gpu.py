@@ -1,10 +1,13 @@
-"""Whole runs against the oracle on device 0: the workloads' step counts on
-grids the oracle finishes in seconds.
+"""Whole runs against the oracle on device 0.
 
+  * the workloads at their full sizes, in the default run: the 128^3
+    three-phase workload over its 2000 steps and the 256^3 headline over 50
+    steps, bit-identical to the oracle (Corey closures, no transcendental
+    calls); the energy model on the 256 x 256 x 128 grid over 20 steps, within
+    1e-12 after the first step and 1e-9 after the last;
   * two-phase Buckley-Leverett at 40^3, 1000 steps (the workload as specified)
-    and the three-phase injection physics at 48^3, 2000 steps (the 128^3
-    workload's step count): every field bit-identical to the oracle after the
-    whole run (Corey closures);
+    and the three-phase injection physics at 48^3, 2000 steps: every field
+    bit-identical to the oracle after the whole run (Corey closures);
   * the three-phase + energy model (reference deck, its closures or Corey-2;
     mu(T) and lambda_g(T) call exp/pow, the device library's, not glibc's)
     at 24 x 20 x 16, 300 steps: every
@@ -163,8 +166,6 @@ def test_energy_full_size_full_run_within_1e9():
     d.close()
 
 
-@pytest.mark.skipif(os.environ.get("CAPAZ_FULL_RUN") != "1",
-                    reason="~1.5 minutes of oracle time on 16 host threads; set CAPAZ_FULL_RUN=1")
 def test_three_phase_128_full_run_is_bit_identical():
     """The 128^3 three-phase workload as specified (2000 steps, 4 Newton
     iterations, Corey-2, lognormal K): every field bit-identical to the
@@ -180,4 +181,47 @@ def test_three_phase_128_full_run_is_bit_identical():
         print(f"128^3 full run: {done + 500} steps bit-identical", file=sys.stderr, flush=True)
     for f, v in od.fields().items():
         assert bit_equal(d.field(f), v), (f, max_rel(d.field(f), v))
+    d.close()
+
+
+def test_headline_full_size_50_steps_are_bit_identical():
+    """The benchmark workload at its full size (three-phase 256^3, Corey-2,
+    lognormal K, top injection): the primary fields bit-identical to the
+    oracle after 1 and 50 fused steps, every field after the 50th."""
+    keys, st = workloads.headline(256)
+    od = oracle.OracleDomain(keys, st)
+    d = capaz.Domain(keys, st)
+    del st
+    for first, count in ((1, 1), (2, 49)):
+        od.run(count, first_step=first)
+        d.run(count, first_step=first)
+        for f in STATE:
+            assert bit_equal(d.field(f), od.field(f)), (first + count - 1, f, max_rel(d.field(f), od.field(f)))
+        print(f"headline 256^3: {first + count - 1} steps bit-identical", file=sys.stderr, flush=True)
+    for f in od.names:
+        g = d.field(f)
+        assert bit_equal(g, od.field(f)), (f, max_rel(g, od.field(f)))
+    d.close()
+
+
+def test_energy_full_size_20_steps_within_bars():
+    """The energy model on its workload's full grid (256 x 256 x 128,
+    reference deck and closures): every field within 1e-12 of the oracle
+    after the first step and within 1e-9 after 20 (relative to the field's
+    magnitude; element-wise for P_w, S_w, S_n)."""
+    keys, st = workloads.energy_3ph(256, 256, 128)
+    od = oracle.OracleDomain(keys, st)
+    d = capaz.Domain(keys, st)
+    del st
+    for first, count, tol in ((1, 1, 1e-12), (2, 19, 1e-9)):
+        od.run(count, first_step=first)
+        d.run(count, first_step=first)
+        worst = 0.0
+        for f in od.names:
+            g, w = d.field(f), od.field(f)
+            r = max_rel(g, w) if f in STATE else norm_rel(g, w)
+            worst = max(worst, r)
+            assert r <= tol, (first + count - 1, f, r)
+        print(f"energy 256x256x128: {first + count - 1} steps, worst relative difference {worst:.2e}",
+              file=sys.stderr, flush=True)
     d.close()

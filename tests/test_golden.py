@@ -12,7 +12,7 @@ from emul import emul
 from golden.make_golden import ENTRIES, load
 from oracle import oracle
 
-GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "ref*.npz")))
 assert GOLDEN, "no golden fixtures"
 
 

@@ -24,7 +24,7 @@ from oracle import oracle
 
 pytestmark = pytest.mark.gpu
 
-GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "ref*.npz")))
 GIDS = [os.path.basename(p)[:-4] for p in GOLDEN]
 TOL_STEP1, TOL_RUN = 1e-12, 1e-9
 
@@ -134,6 +134,47 @@ def test_headline_physics_bitwise_at_96():
     d.run(2, first_step=2)
     for f in od.names:
         assert bit_equal(d.field(f), od.field(f)), (f, max_rel(d.field(f), od.field(f)))
+
+
+def pivot_search_state(n: int = 40):
+    """The benchmark physics (three-phase, beta_w 4.5e-10, Corey-2) with
+    S_w in [0.005, 0.02] everywhere: the 3x3 Newton matrix's first pivot
+    ro0_w * beta_w * S_w is below 1e-8 in every interior cell, so the inverse
+    takes gauss.cpp:39-58's signed row search (both later rows qualify, so
+    row 0 is swapped twice)."""
+    keys, st = workloads.three_phase_injection(n, seed=11)
+    rng = np.random.default_rng(12)
+    st = dict(st)
+    st["S_w"] = rng.uniform(0.005, 0.02, st["S_w"].shape)
+    st["S_n"] = rng.uniform(0.6, 0.8, st["S_n"].shape)
+    inner = (slice(1, -1),) * 3
+    dF0 = keys["RO_W"] * keys["BETA_W"] * st["S_w"][inner]
+    dF3 = keys["RO_N"] * keys["BETA_N"] * st["S_n"][inner]
+    dF6 = keys["RO_G"] * (1.0 - st["S_w"][inner] - st["S_n"][inner]) / keys["P_ATM"]
+    assert (dF0 < 1e-8).all() and (dF3 > 1e-8).all() and (dF6 > 1e-8).all()
+    return keys, st
+
+
+def test_three_phase_pivot_search_path_bitwise():
+    """Every interior cell's first Newton iteration takes the signed pivot
+    search (the out-of-line general routine on the device): three fused
+    steps and three entry-point sequences equal the oracle bit for bit,
+    every field."""
+    keys, st = pivot_search_state()
+    od = oracle.OracleDomain(keys, st)
+    fused = capaz.Domain(keys, st)
+    seq = capaz.Domain(keys, st)
+    od.run(3)
+    fused.run(3)
+    for t in range(1, 4):
+        seq.time_step_function(t * keys["DT"])
+    assert np.isfinite(od.field("P_w")).all()
+    for f in od.names:
+        w = od.field(f)
+        assert bit_equal(fused.field(f), w), (f, max_rel(fused.field(f), w))
+        assert bit_equal(seq.field(f), w), (f, max_rel(seq.field(f), w))
+    fused.close()
+    seq.close()
 
 
 def test_energy_workload_vs_oracle():
