@@ -53,6 +53,8 @@ EXPORTS = {
     "capaz_get_tuning": (ctypes.c_int, ["ctx", ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
     "capaz_kernel_time": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)]),
     "capaz_check_finite": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_int64)]),
+    "capaz_check_finite_noalloc": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int64,
+                                                  ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]),
     "capaz_sync": (ctypes.c_int, ["ctx"]),
     "capaz_stream": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_void_p)]),
     "capaz_load_face": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),

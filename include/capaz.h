@@ -207,6 +207,23 @@ int capaz_kernel_time(capaz_ctx* ctx, double* total_ms, long* launches);
  * checks of test_nan (shared_test.cpp:56-70), which never stop a run. */
 int capaz_check_finite(capaz_ctx* ctx, int64_t* first_bad);
 
+/* The same sentinel over caller-owned device arrays, allocating nothing.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc); it needs no context.  fields: a host array of nfields
+ * (1..CAPAZ_SENTINEL_MAX_FIELDS) device pointers of n doubles each, on the
+ * current device; stream: a hipStream_t (NULL: the null stream), which the
+ * call waits for.  CAPAZ_OK and *first_bad = -1 when every value is finite;
+ * else CAPAZ_ERR_NONFINITE and *first_bad = the smallest index holding a NaN
+ * or an infinity in some field (capaz_last_error(NULL) names the fields).
+ * Its only device memory is a static array of CAPAZ_SENTINEL_MAX_FIELDS
+ * result slots per GPU (held until the process ends), shared by every call on
+ * that GPU: calls that overlap on one GPU race; nfields above the maximum is
+ * rejected with CAPAZ_ERR_INVALID. */
+#define CAPAZ_SENTINEL_MAX_FIELDS 8
+int capaz_check_finite_noalloc(const double* const* fields, int nfields, int64_t n, void* stream,
+                               int64_t* first_bad);
+
 /* Waits for all work queued on the context. */
 int capaz_sync(capaz_ctx* ctx);
 

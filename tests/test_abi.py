@@ -74,3 +74,16 @@ def test_null_context_calls_fail_cleanly():
     assert L.capaz_step(None, 1, 1) == -1
     assert L.capaz_destroy(None) == 0
     assert L.capaz_cells(None) == 0
+
+
+def test_noalloc_sentinel_validates_before_any_device_call():
+    L = capaz.load()
+    first = ctypes.c_int64(7)
+    ptrs = (ctypes.c_void_p * 9)(*([16] * 9))
+    assert L.capaz_check_finite_noalloc(ptrs, 9, 10, None, ctypes.byref(first)) == -1
+    assert "outside 1..8" in L.capaz_last_error(None).decode()
+    assert L.capaz_check_finite_noalloc(ptrs, 0, 10, None, ctypes.byref(first)) == -1
+    assert L.capaz_check_finite_noalloc(None, 1, 10, None, ctypes.byref(first)) == -1
+    assert L.capaz_check_finite_noalloc(ptrs, 1, -1, None, ctypes.byref(first)) == -1
+    # nothing to scan: finite, without touching a device
+    assert L.capaz_check_finite_noalloc(ptrs, 1, 0, None, ctypes.byref(first)) == 0 and first.value == -1
