@@ -18,9 +18,24 @@
 // reference CPU path's arithmetic (cpu.cpp + model .cpp + energy.cpp) on the
 // device, so results match the CPU build, not gpu.cu's divergent
 // three-phase closures.
+//
+// Fused steps.  time_step_function (main.cpp:93-108) calls the step's entry
+// points in a fixed order.  The adapter records them instead of running
+// them; when a call completes the order (boundary_conditions), it runs the
+// whole step as one capaz_step -- the fused kernels, the same bits as the
+// eight entry points.  Any other call (data transfers, face exchanges) or a
+// call out of order first runs what was recorded, one entry point at a time.
+// A fused step ping-pongs the primary state between two device arrays, so
+// the adapter maps the pointers it handed to the driver onto field ids and
+// moves data with capaz_get_field / capaz_set_field, which read and write the
+// current arrays (recomputing the intermediate fields where needed).
+// CAPAZ_DROPIN_FUSED=0 in the environment runs every entry point as called.
 #include "defines.h"
 
 #include <hip/hip_runtime.h>
+
+#include <cstdlib>
+#include <cstring>
 
 #include "capaz.h"
 
@@ -28,6 +43,21 @@ namespace {
 
 capaz_ctx* g_ctx = nullptr;
 consts g_def;
+// The device pointers handed to the driver at allocation, by field id.
+double* g_orig[CAPAZ_NUM_FIELDS] = {};
+
+// The step's entry points in time_step_function's order (main.cpp:93-108).
+enum Hot { H_S, H_PXR, H_U, H_ROS, H_HE, H_E, H_NEWTON, H_BOUNDARY };
+#ifdef ENERGY
+const Hot kOrder[] = {H_S, H_PXR, H_U, H_ROS, H_HE, H_E, H_NEWTON, H_BOUNDARY};
+#else
+const Hot kOrder[] = {H_S, H_PXR, H_U, H_ROS, H_NEWTON, H_BOUNDARY};
+#endif
+constexpr int kSteps = sizeof(kOrder) / sizeof(kOrder[0]);
+bool g_fuse = true;
+int g_pending = 0;   // entries of kOrder recorded and not yet run
+double g_t = 0.;     // t of the recorded roS_calculation
+long g_fused = 0;    // steps run as capaz_step
 // The driver's host arrays, as last passed to an entry point.  save_data_plots
 // downloads neither S_g nor T (main.cpp:478-491) and initialization uploads
 // no T (main.cpp:329-339) -- gpu.cu had no energy step and its plots showed
@@ -59,6 +89,49 @@ double* dev_field(int id) {
     double* p = nullptr;
     check(capaz_field_device_ptr(g_ctx, id, &p), "capaz_field_device_ptr");
     return p;
+}
+
+// The field id of a device pointer the driver holds, -1 if it is not one.
+int field_of(const double* dev) {
+    for (int f = 0; f < CAPAZ_NUM_FIELDS; ++f)
+        if (g_orig[f] && g_orig[f] == dev) return f;
+    return -1;
+}
+
+void run_one(Hot h, double t) {
+    switch (h) {
+        case H_S: check(capaz_S(g_ctx), "capaz_S"); break;
+        case H_PXR: check(capaz_P_Xi_ro(g_ctx), "capaz_P_Xi_ro"); break;
+        case H_U: check(capaz_u(g_ctx), "capaz_u"); break;
+        case H_ROS: check(capaz_roS(g_ctx, t), "capaz_roS"); break;
+        case H_HE: check(capaz_energy_HE(g_ctx), "capaz_energy_HE"); break;
+        case H_E: check(capaz_energy_E(g_ctx), "capaz_energy_E"); break;
+        case H_NEWTON: check(capaz_newton(g_ctx), "capaz_newton"); break;
+        case H_BOUNDARY: check(capaz_boundary(g_ctx), "capaz_boundary"); break;
+    }
+}
+
+// Runs the recorded entry points one at a time.
+void flush() {
+    for (int q = 0; q < g_pending; ++q) run_one(kOrder[q], g_t);
+    g_pending = 0;
+}
+
+// An entry point of the step: recorded while the calls follow
+// time_step_function's order, the whole step run as one fused capaz_step when
+// the order completes.
+void hot(Hot h, double t = 0.) {
+    if (g_fuse && kOrder[g_pending] == h) {
+        if (h == H_ROS) g_t = t;
+        if (++g_pending == kSteps) {
+            g_pending = 0;
+            // capaz_step's t = step * dt is unused: assign_roS_nr ignores t
+            check(capaz_step(g_ctx, ++g_fused, 1), "capaz_step");
+        }
+        return;
+    }
+    flush();
+    run_one(h, t);
 }
 
 capaz_params params_of(const consts& def) {
@@ -109,9 +182,8 @@ void face(double* DevArrayPtr, double* HostBuffer, double* DevBuffer, int axis, 
           const consts& def) {
     size_t plane = axis == 0 ? (size_t)def.locNy * def.locNz
                              : (axis == 1 ? (size_t)def.locNx * def.locNz : (size_t)def.locNx * def.locNy);
-    int field = -1;
-    for (int f = 0; f < CAPAZ_NUM_FIELDS && field < 0; ++f)
-        if (capaz_has_field(g_ctx, f) && dev_field(f) == DevArrayPtr) field = f;
+    flush();
+    const int field = field_of(DevArrayPtr);
     if (field < 0) print_error((char*)"face exchange on an array the backend does not own", (char*)__FILE__, __LINE__);
     if (save) {
         check(hipMemcpy(DevBuffer, HostBuffer, plane * sizeof(double), hipMemcpyHostToDevice) == hipSuccess
@@ -147,6 +219,8 @@ void device_initialization(consts* def) {
                     __LINE__);
     def->blocksX = def->blocksY = def->blocksZ = 0;  // the backend chooses its own launch shapes
     g_def = *def;
+    const char* fuse = std::getenv("CAPAZ_DROPIN_FUSED");
+    g_fuse = !(fuse && std::strcmp(fuse, "0") == 0);
 }
 
 void device_finalization(void) {}
@@ -179,22 +253,24 @@ void device_memory_allocation(ptr_Arrays* ArraysPtr, double** DevBuffer, consts 
         CAPAZ_T, CAPAZ_H_w, CAPAZ_H_n, CAPAZ_H_g, CAPAZ_H_r, CAPAZ_E, CAPAZ_E_new,
 #endif
     };
-    for (size_t f = 0; f < sizeof(ids) / sizeof(ids[0]); ++f) *slots[f] = dev_field(ids[f]);
+    for (size_t f = 0; f < sizeof(ids) / sizeof(ids[0]); ++f) *slots[f] = g_orig[ids[f]] = dev_field(ids[f]);
     *DevBuffer = nullptr;  // one domain: no exchange buffer (gpu.cu:676-685 sizes it from sizex/y/z)
 }
 
 void device_memory_free(ptr_Arrays DevArraysPtr, double* DevBuffer) {
     (void)DevArraysPtr;
     (void)DevBuffer;
+    flush();
     capaz_destroy(g_ctx);
     g_ctx = nullptr;
 }
 
 void load_data_to_host(double* HostArrayPtr, double* DevArrayPtr, consts def) {
-    check(capaz_sync(g_ctx), "capaz_sync");
-    if (hipMemcpy(HostArrayPtr, DevArrayPtr, cells(def) * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
-        print_error((char*)"copy data to host", (char*)__FILE__, __LINE__);
-    if (g_have_host && DevArrayPtr == dev_field(CAPAZ_P_w)) {
+    flush();
+    const int id = field_of(DevArrayPtr);
+    if (id < 0) print_error((char*)"load_data_to_host: not a backend array", (char*)__FILE__, __LINE__);
+    check(capaz_get_field(g_ctx, id, HostArrayPtr, cells(def)), "capaz_get_field");
+    if (g_have_host && id == CAPAZ_P_w) {
 #ifdef THREE_PHASE
         check(capaz_get_field(g_ctx, CAPAZ_S_g, g_host.S_g, cells(def)), "capaz_get_field(S_g)");
 #endif
@@ -205,12 +281,14 @@ void load_data_to_host(double* HostArrayPtr, double* DevArrayPtr, consts def) {
 }
 
 void load_data_to_device(double* HostArrayPtr, double* DevArrayPtr, consts def) {
-    check(capaz_sync(g_ctx), "capaz_sync");
-    if (hipMemcpy(DevArrayPtr, HostArrayPtr, cells(def) * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-        print_error((char*)"copy double data to device", (char*)__FILE__, __LINE__);
+    flush();
+    const int id = field_of(DevArrayPtr);
+    if (id < 0) print_error((char*)"load_data_to_device: not a backend array", (char*)__FILE__, __LINE__);
+    check(capaz_set_field(g_ctx, id, HostArrayPtr, cells(def)), "capaz_set_field");
 }
 
 void load_data_to_device_int(int* HostArrayPtr, int* DevArrayPtr, consts def) {
+    flush();
     check(capaz_sync(g_ctx), "capaz_sync");
     if (hipMemcpy(DevArrayPtr, HostArrayPtr, cells(def) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
         print_error((char*)"copy int data to device", (char*)__FILE__, __LINE__);
@@ -220,32 +298,21 @@ void S_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts def
     remember_host(HostArraysPtr);
 #ifdef ENERGY
     if (!g_T_on_device) {
+        flush();
         check(capaz_set_field(g_ctx, CAPAZ_T, HostArraysPtr.T, cells(def)), "capaz_set_field(T)");
         g_T_on_device = true;
     }
 #endif
-    check(capaz_S(g_ctx), "capaz_S");
+    hot(H_S);
 }
-void ro_P_Xi_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts def) {
-    check(capaz_P_Xi_ro(g_ctx), "capaz_P_Xi_ro");
-}
-void u_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts def) { check(capaz_u(g_ctx), "capaz_u"); }
-void roS_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, double t, consts def) {
-    check(capaz_roS(g_ctx, t), "capaz_roS");
-}
-void P_S_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts def) {
-    check(capaz_newton(g_ctx), "capaz_newton");
-}
-void boundary_conditions(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts def) {
-    check(capaz_boundary(g_ctx), "capaz_boundary");
-}
+void ro_P_Xi_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts def) { hot(H_PXR); }
+void u_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts def) { hot(H_U); }
+void roS_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, double t, consts def) { hot(H_ROS, t); }
+void P_S_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts def) { hot(H_NEWTON); }
+void boundary_conditions(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts def) { hot(H_BOUNDARY); }
 #ifdef ENERGY
-void H_E_current_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts def) {
-    check(capaz_energy_HE(g_ctx), "capaz_energy_HE");
-}
-void E_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts def) {
-    check(capaz_energy_E(g_ctx), "capaz_energy_E");
-}
+void H_E_current_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts def) { hot(H_HE); }
+void E_calculation(ptr_Arrays HostArraysPtr, ptr_Arrays DevArraysPtr, consts def) { hot(H_E); }
 #endif
 
 #define CAPAZ_FACE(name, axis, side, save)                                                             \

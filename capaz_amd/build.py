@@ -58,7 +58,7 @@ def _run(cmd: list[str]) -> None:
 def build_library() -> str:
     srcs = [os.path.join(CSRC, f) for f in
             ("capaz_kernels.hip", "capaz_batch.hip", "capaz_adjoint.hip", "capaz_api.cpp", "capaz_batch_api.cpp",
-             "capaz_adjoint_api.cpp", "capaz_state.cpp", "capaz_plot.cpp", "capaz_noalloc.hip")]
+             "capaz_adjoint_api.cpp", "capaz_state.cpp", "capaz_plot.cpp", "capaz_noalloc.hip", "capaz_energy2.hip")]
     tmp = LIB + ".tmp"
     cmd = [_hipcc(), *HIP_FLAGS, "-x", "hip", *srcs, "-o", tmp]
     _run(cmd)

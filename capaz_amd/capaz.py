@@ -59,6 +59,10 @@ EXPORTS = {
     "capaz_stream": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_void_p)]),
     "capaz_load_face": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
     "capaz_save_face": (ctypes.c_int, ["ctx", ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]),
+    "capaz_load_faces": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p]),
+    "capaz_save_faces": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_void_p]),
     "capaz_save_state": (ctypes.c_int, ["ctx", ctypes.c_char_p, ctypes.c_long]),
     "capaz_write_plot_rows": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.c_long,
                                              ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
@@ -96,6 +100,10 @@ class Params(ctypes.Structure):
         ("S_wr", ctypes.c_double), ("S_nr", ctypes.c_double), ("S_gr", ctypes.c_double),
         ("P_d", ctypes.c_double), ("S_w_gr", ctypes.c_double), ("S_n_gr", ctypes.c_double),
         ("corey_n", ctypes.c_double),
+        ("P_bottom", ctypes.c_double), ("energy_closure", ctypes.c_int),
+        ("lambda_eff", ctypes.c_double), ("c_r", ctypes.c_double), ("ro_r", ctypes.c_double),
+        ("T_top", ctypes.c_double), ("T_bottom", ctypes.c_double),
+        ("mu_T_a", ctypes.c_double), ("mu_T_ref", ctypes.c_double),
     ]
 
 
@@ -151,7 +159,9 @@ def load() -> ctypes.CDLL:
 
 def params_from_keys(keys: dict, device: int = 0) -> Params:
     """capaz_params from defines.ini keys (main.cpp:1327-1626) plus MODEL,
-    ENERGY, CLOSURE ("reference"/"corey") and COREY_N."""
+    ENERGY, CLOSURE ("reference"/"corey"), COREY_N, P_BOTTOM and the
+    configurations' energy closure (ENERGY_CLOSURE "config", LAMBDA_EFF, C_R,
+    RO_R, T_TOP, T_BOTTOM, MU_T_A, MU_T_REF)."""
     p = Params()
     p.struct_size = ctypes.sizeof(Params)
     g = lambda k, d=0.0: float(keys.get(k, d))
@@ -172,6 +182,11 @@ def params_from_keys(keys: dict, device: int = 0) -> Params:
     p.P_d = g("P_D_0")
     p.S_w_gr, p.S_n_gr = g("S_W_GR"), g("S_N_GR")
     p.corey_n = g("COREY_N", 2.0)
+    p.P_bottom = g("P_BOTTOM")
+    p.energy_closure = 1 if keys.get("ENERGY_CLOSURE", "reference") == "config" else 0
+    p.lambda_eff, p.c_r, p.ro_r = g("LAMBDA_EFF"), g("C_R"), g("RO_R")
+    p.T_top, p.T_bottom = g("T_TOP"), g("T_BOTTOM")
+    p.mu_T_a, p.mu_T_ref = g("MU_T_A"), g("MU_T_REF")
     return p
 
 
@@ -275,7 +290,7 @@ class Domain:
 
     def tune(self, **kw) -> None:
         """Launch shape of the fused step (capaz_tune): fused_kernel, tile_y,
-        kchunk, adjoint_kernel, energy_kernel, energy_newton; timing=1 records events around
+        kchunk, adjoint_kernel, adjoint_streams, energy_kernel, energy_newton; timing=1 records events around
         the main kernel of each step; check_every=K runs the non-finite
         sentinel after every K-th step."""
         for k, v in kw.items():
@@ -283,7 +298,7 @@ class Domain:
 
     def tuning(self) -> dict:
         out = {}
-        for k in ("check_every", "fused_kernel", "tile_y", "kchunk", "adjoint_kernel", "energy_kernel",
+        for k in ("check_every", "fused_kernel", "tile_y", "kchunk", "adjoint_kernel", "adjoint_streams", "energy_kernel",
                   "energy_newton", "timing", "adjoint_segment"):
             v = ctypes.c_int()
             self._check(load().capaz_get_tuning(self._ctx, k.encode(), ctypes.byref(v)), "capaz_get_tuning")

@@ -79,6 +79,7 @@ int workspace(capaz_ctx* c, size_t doubles, double** out) {
     const size_t bytes = doubles * sizeof(double);
     if (c->adj_bytes < bytes) {
         ADJ_HIP(c, hipStreamSynchronize(c->stream));
+        if (c->adj_stream) ADJ_HIP(c, hipStreamSynchronize(c->adj_stream));
         if (c->adj_mem) (void)hipFree(c->adj_mem);
         c->adj_mem = nullptr;
         c->adj_bytes = 0;
@@ -90,8 +91,9 @@ int workspace(capaz_ctx* c, size_t doubles, double** out) {
 }
 
 // One fused step from the five arrays at `from` to those at `to` (stride
-// apart); with div != NULL it also records the flux divergence (3 fields).
-int fused_between(capaz_ctx* c, const double* from, double* to, size_t stride, double* div = nullptr) {
+// apart) on stream s; with div != NULL it also records the flux divergence
+// (3 fields).
+int fused_between(capaz_ctx* c, hipStream_t s, const double* from, double* to, size_t stride, double* div = nullptr) {
     capaz::StepIO io;
     io.Pw = from;
     io.Sw = from + stride;
@@ -107,9 +109,9 @@ int fused_between(capaz_ctx* c, const double* from, double* to, size_t stride, d
     io.Pn1 = to + 3 * stride;
     io.Pg1 = to + 4 * stride;
     if (div)
-        ADJ_HIP(c, capaz::launch_fused_record(c->p, io, c->tuning.kchunk, div, (int64_t)stride, c->stream));
+        ADJ_HIP(c, capaz::launch_fused_record(c->p, io, c->tuning.kchunk, div, (int64_t)stride, s));
     else
-        ADJ_HIP(c, capaz::launch_fused(c->p, io, c->tuning, c->stream));
+        ADJ_HIP(c, capaz::launch_fused(c->p, io, c->tuning, s));
     return CAPAZ_OK;
 }
 
@@ -155,7 +157,8 @@ int forward(capaz_ctx* c, long nsteps, const Obs& ob, const int64_t* d_cells, do
         return t == rec_from ? const_cast<double*>(x0) : rec_seg + (size_t)(t - rec_from) * kState * stride;
     };
     for (long t = rec_from; t < nsteps; ++t) {
-        if ((rc = fused_between(c, slot(t), slot(t + 1), stride, rec_div + (size_t)(t - rec_from) * 3 * stride)))
+        if ((rc = fused_between(c, c->stream, slot(t), slot(t + 1), stride,
+                                rec_div + (size_t)(t - rec_from) * 3 * stride)))
             return rc;
         if ((rc = gather(t + 1, slot(t + 1) + stride, slot(t + 1) + 2 * stride))) return rc;
     }
@@ -306,15 +309,18 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
     if (c->m_has_zero) return fail(c, CAPAZ_ERR_STATE, "adjoint: cells with m == 0");
     const size_t S = field_stride(c->p.n);
     const size_t n_state = (size_t)kState * S;
-    // layout: checkpoints | segment states (C + 1) | recorded divergence (C x 3) | lambda x2 | work (6) |
-    // grads (2) | seen | observed | cells
+    // layout: checkpoints | segment buffers (1, or 2 with two streams and
+    // more than one segment): states (C + 1) | recorded divergence (C x 3) |
+    // lambda x2 | work (6) | grads (2) | seen | observed | cells
     const bool recorded = c->tuning.adjoint_kernel == 2;
     const size_t ns = (size_t)ob.nsamples * 2 * ob.ncells;
     const size_t n_work = 6 * S;
+    auto buffers_for = [&](long seg) { return c->tuning.adjoint_streams == 2 && seg < nsteps ? 2 : 1; };
     auto doubles_for = [&](long seg) {
         const long segs = (nsteps + seg - 1) / seg;
-        return (size_t)segs * n_state + (size_t)(seg + 1) * n_state + (recorded ? (size_t)seg * 3 * S : 0) +
-               2 * n_state + n_work + 2 * S + 2 * ns + (size_t)ob.ncells;
+        const size_t per_buf = (size_t)(seg + 1) * n_state + (recorded ? (size_t)seg * 3 * S : 0);
+        return (size_t)segs * n_state + (size_t)buffers_for(seg) * per_buf + 2 * n_state + n_work + 2 * S + 2 * ns +
+               (size_t)ob.ncells;
     };
     // Segment length C: checkpoint_every if > 0; round(sqrt(nsteps)) if < 0;
     // if 0, the whole trajectory as one segment (recorded by the forward
@@ -345,11 +351,17 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
     if (rc) return rc;
     c->adj_segment = C;
     const long nseg = (nsteps + C - 1) / C;
+    const int nbuf = buffers_for(C);
     const size_t n_div = recorded ? (size_t)C * 3 * S : 0;
     double* ckpt = w;
-    double* seg = ckpt + (size_t)nseg * n_state;
-    double* divs = seg + (size_t)(C + 1) * n_state;
-    double* lam[2] = {divs + n_div, divs + n_div + n_state};
+    double* seg[2];  // segment buffer b: C + 1 states, then C recorded divergences
+    double* divs[2];
+    seg[0] = ckpt + (size_t)nseg * n_state;
+    divs[0] = seg[0] + (size_t)(C + 1) * n_state;
+    seg[1] = divs[0] + n_div;
+    divs[1] = seg[1] + (size_t)(C + 1) * n_state;
+    double* const bufs_end = nbuf == 2 ? divs[1] + n_div : seg[1];
+    double* lam[2] = {bufs_end, bufs_end + n_state};
     double* work = lam[1] + n_state;
     double* grad = work + n_work;
     double* seen = grad + 2 * S;
@@ -367,7 +379,8 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
     // forward sweep (objective_value waits for it, so `want` outlives its copy);
     // recording, it runs the last segment the way the backward sweep would
     const long last0 = (nseg - 1) * C;
-    if ((rc = recorded ? forward(c, nsteps, ob, d_cells, seen, C, ckpt, S, last0, seg, divs)
+    const int last_buf = (int)((nseg - 1) % nbuf);
+    if ((rc = recorded ? forward(c, nsteps, ob, d_cells, seen, C, ckpt, S, last0, seg[last_buf], divs[last_buf])
                        : forward(c, nsteps, ob, d_cells, seen, C, ckpt, S)))
         return rc;
     if ((rc = objective_value(c, o, ob, seen, J))) return rc;
@@ -415,21 +428,51 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
         a.g_m = grad + S;
         return a;
     };
+    // Segment q lives in buffer q % nbuf.  With two buffers the recompute of
+    // segment q - 1 runs on the second stream while the passes reverse
+    // segment q; events order each recompute after the passes that last read
+    // its buffer, and the passes after the recompute of their segment.  The
+    // kernels and their inputs are those of one stream: the bits are the same.
+    hipStream_t s2 = c->stream;
+    if (nbuf == 2) {
+        if (!c->adj_stream) ADJ_HIP(c, hipStreamCreateWithFlags(&c->adj_stream, hipStreamNonBlocking));
+        for (hipEvent_t& ev : c->adj_ev)
+            if (!ev) ADJ_HIP(c, hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        s2 = c->adj_stream;
+        // both buffers free once the forward sweep and the memsets are done
+        for (int b = 0; b < 2; ++b) ADJ_HIP(c, hipEventRecord(c->adj_ev[2 + b], c->stream));
+    }
+    // the segment's states (and, recording, every step's divergence: the
+    // last step's new state lands in the spare slot C)
+    auto slot_of = [&](long q, long t) {
+        const long t0 = q * C;
+        return t == t0 ? ckpt + (size_t)q * n_state : seg[q % nbuf] + (size_t)(t - t0) * n_state;
+    };
+    auto recompute = [&](long q, hipStream_t s) -> int {
+        const long t0 = q * C, t1 = std::min(nsteps, t0 + C);
+        double* dv = divs[q % nbuf];
+        for (long t = t0; t + 1 < t1 || (recorded && t < t1); ++t)
+            if ((rc = fused_between(c, s, slot_of(q, t), slot_of(q, t + 1), S,
+                                    recorded ? dv + (size_t)(t - t0) * 3 * S : nullptr)))
+                return rc;
+        return CAPAZ_OK;
+    };
+    if (!recorded && (rc = recompute(nseg - 1, c->stream))) return rc;  // else recorded by the forward sweep
     int cur = 0;
     for (long q = nseg - 1; q >= 0; --q) {
         const long t0 = q * C, t1 = std::min(nsteps, t0 + C);
-        auto slot = [&](long t) { return t == t0 ? ckpt + (size_t)q * n_state : seg + (size_t)(t - t0) * n_state; };
-        // the segment's states (and, recording, every step's divergence: the
-        // last step's new state lands in the spare slot C)
-        const bool have = recorded && q == nseg - 1;  // recorded by the forward sweep
-        for (long t = t0; !have && (t + 1 < t1 || (recorded && t < t1)); ++t)
-            if ((rc = fused_between(c, slot(t), slot(t + 1), S, recorded ? divs + (size_t)(t - t0) * 3 * S : nullptr)))
-                return rc;
+        if (q >= 1 && nbuf == 2) {
+            const int b = (int)((q - 1) % 2);
+            ADJ_HIP(c, hipStreamWaitEvent(s2, c->adj_ev[2 + b], 0));
+            if ((rc = recompute(q - 1, s2))) return rc;
+            ADJ_HIP(c, hipEventRecord(c->adj_ev[b], s2));
+        }
+        double* dv = divs[q % nbuf];
         for (long t = t1 - 1; t >= t0; --t) {
             double* l0 = lam[cur ^ 1];
-            const AdjIO a = io_at(slot(t), work, lam[cur], l0);
+            const AdjIO a = io_at(slot_of(q, t), work, lam[cur], l0);
             if (recorded)
-                e = launch_adjoint_step_recorded(c->p, a, divs + (size_t)(t - t0) * 3 * S, (int64_t)S, c->stream);
+                e = launch_adjoint_step_recorded(c->p, a, dv + (size_t)(t - t0) * 3 * S, (int64_t)S, c->stream);
             else
                 e = c->tuning.adjoint_kernel == 1 ? launch_adjoint_step_tiled(c->p, a, c->stream)
                                                   : launch_adjoint_step(c->p, a, c->stream);
@@ -437,6 +480,12 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
                 return hip_fail(c, e, "launch_adjoint_step");
             if (t >= 1 && (rc = obs_adjoint(l0, t))) return rc;
             cur ^= 1;
+        }
+        if (q >= 1 && nbuf == 2) {
+            ADJ_HIP(c, hipEventRecord(c->adj_ev[2 + q % 2], c->stream));
+            ADJ_HIP(c, hipStreamWaitEvent(c->stream, c->adj_ev[(q - 1) % 2], 0));
+        } else if (q >= 1 && (rc = recompute(q - 1, c->stream))) {
+            return rc;
         }
     }
     const size_t bytes = (size_t)c->p.n * sizeof(double);

@@ -10,7 +10,10 @@
 // interior P_n / P_g); they are marked stale and recomputed from the previous
 // state -- which the second arrays still hold -- before anything reads them,
 // so every field always reads as after the same number of reference steps.
-// The energy model's step (six passes) works in place like the sequence.
+// The three-phase energy model's step (six passes) works in place like the
+// sequence; the two-phase energy model's fused step ping-pongs P_w, S_n and T
+// and re-runs the step's entry points from the previous state when a caller
+// needs the intermediates.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -62,6 +65,7 @@ using capaz::detail::hip_fail;
 
 bool field_exists(const capaz_params& u, int f) {
     if (f < 0 || f >= capaz::F_COUNT) return false;
+    if (f == capaz::F_H_g) return u.energy == 1 && u.model == 3;
     if (f >= capaz::F_T) return u.energy == 1;
     if (f >= capaz::F_S_g) return u.model == 3;
     return true;
@@ -96,6 +100,18 @@ int materialize(capaz_ctx* c) {
     for (int f = 0; f < capaz::F_COUNT; ++f)
         if (c->primary[f]) F.f[f] = c->alt[f];
     int rc;
+    if (c->user.energy && c->user.model == 2) {
+        // two-phase energy model: the Newton's S_w, H and E are intermediates
+        // too, so the whole step runs again from the previous state; its new
+        // state lands in the second arrays with the bits the current ones hold
+        using capaz::Entry;
+        for (Entry e : {Entry::S, Entry::P_Xi_ro, Entry::U, Entry::RoS, Entry::HE, Entry::E})
+            if ((rc = run_entry(c, e, F))) return rc;
+        CAPAZ_HIP(c, capaz::launch_newton_energy(c->p, F, c->tuning.energy_newton ? c->d_redo : nullptr, c->stream));
+        if ((rc = run_entry(c, Entry::Boundary, F))) return rc;
+        c->stale = false;
+        return CAPAZ_OK;
+    }
     if ((rc = run_entry(c, capaz::Entry::S, F))) return rc;
     if ((rc = run_entry(c, capaz::Entry::P_Xi_ro, F))) return rc;
     if ((rc = run_entry(c, capaz::Entry::U, F))) return rc;
@@ -264,7 +280,8 @@ int capaz_create(const capaz_params* u, capaz_ctx** out) {
         if (!field_exists(*u, f)) continue;
         if ((e = hipMalloc(&c->cur[f], bytes)) != hipSuccess) return bail(e, "hipMalloc");
         if ((e = hipMemsetAsync(c->cur[f], 0, bytes, c->stream)) != hipSuccess) return bail(e, "hipMemset");
-        c->primary[f] = is_primary(*u, f) && !u->energy;
+        // ping-pong: the fused steps (the three-phase energy step works in place)
+        c->primary[f] = is_primary(*u, f) && !(u->energy && u->model == 3);
         if (c->primary[f]) {
             if ((e = hipMalloc(&c->alt[f], bytes)) != hipSuccess) return bail(e, "hipMalloc");
             if ((e = hipMemsetAsync(c->alt[f], 0, bytes, c->stream)) != hipSuccess) return bail(e, "hipMemset");
@@ -288,7 +305,11 @@ int capaz_destroy(capaz_ctx* c) {
     if (c->d_dtm) (void)hipFree(c->d_dtm);
     if (c->d_lam) (void)hipFree(c->d_lam);
     if (c->d_redo) (void)hipFree(c->d_redo);
+    if (c->adj_stream) (void)hipStreamSynchronize(c->adj_stream);
     if (c->adj_mem) (void)hipFree(c->adj_mem);
+    for (hipEvent_t e : c->adj_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->adj_stream) (void)hipStreamDestroy(c->adj_stream);
     for (auto& e : c->ev) {
         if (e.begin) (void)hipEventDestroy(e.begin);
         if (e.end) (void)hipEventDestroy(e.end);
@@ -368,6 +389,7 @@ int capaz_energy_E(capaz_ctx* c) {
     if ((rc = check_ctx(c))) return rc;
     if (!c->user.energy) return fail(c, CAPAZ_ERR_STATE, "capaz_energy_E: not in this model");
     if ((rc = materialize(c))) return rc;
+    if (c->user.model == 2) return run_entry(c, capaz::Entry::E, fields_of(c));
     // on the z tiles, lambda_eff evaluated once per staged cell
     CAPAZ_HIP(c, capaz::launch_energy_e_tiles(c->p, fields_of(c), nullptr, c->stream));
     return CAPAZ_OK;
@@ -421,7 +443,7 @@ int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
     if (nsteps < 0) return fail(c, CAPAZ_ERR_INVALID, "nsteps must be >= 0");
     if ((rc = scan_m(c))) return rc;
     using namespace capaz;
-    if (c->user.energy && !c->m_has_zero && c->tuning.energy_kernel == 1) {
+    if (c->user.energy && c->user.model == 3 && !c->m_has_zero && c->tuning.energy_kernel >= 1) {
         // the energy model's six passes, in place like the sequence
         if ((rc = materialize(c))) return rc;
         for (long s = 0; s < nsteps; ++s) {
@@ -429,6 +451,21 @@ int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
             if ((rc = next_events(c, &ev))) return rc;
             CAPAZ_HIP(c, launch_energy_step(c->p, fields_of(c), c->d_lam, c->d_dtm,
                                             c->tuning.energy_newton ? c->d_redo : nullptr, c->stream, ev));
+            if ((rc = check_after_step(c, first_step + s))) return rc;
+        }
+        return CAPAZ_OK;
+    }
+    if (c->user.energy && c->user.model == 2 && !c->m_has_zero && c->tuning.energy_kernel >= 1) {
+        // the two-phase energy model's fused step (ping-pong P_w, S_n, T)
+        for (long s = 0; s < nsteps; ++s) {
+            const double* cur[5] = {c->cur[F_P_w], c->cur[F_S_n], c->cur[F_T], c->cur[F_m], c->cur[F_K]};
+            double* next[3] = {c->alt[F_P_w], c->alt[F_S_n], c->alt[F_T]};
+            const KernelEvents* ev;
+            if ((rc = next_events(c, &ev))) return rc;
+            CAPAZ_HIP(c, launch_energy2_step(c->p, cur, next, c->tuning.kchunk, c->stream, ev));
+            for (int f = 0; f < F_COUNT; ++f)
+                if (c->primary[f]) std::swap(c->cur[f], c->alt[f]);
+            c->stale = true;
             if ((rc = check_after_step(c, first_step + s))) return rc;
         }
         return CAPAZ_OK;
@@ -481,6 +518,8 @@ int capaz_tune(capaz_ctx* c, const char* key, int value) {
         c->tuning.kchunk = value;
     else if (k == "adjoint_kernel" && value >= 0 && value <= 2)
         c->tuning.adjoint_kernel = value;
+    else if (k == "adjoint_streams" && (value == 1 || value == 2))
+        c->tuning.adjoint_streams = value;
     else if (k == "energy_kernel" && (value == 0 || value == 1))
         c->tuning.energy_kernel = value;
     else if (k == "energy_newton" && (value == 0 || value == 1))
@@ -504,6 +543,7 @@ int capaz_get_tuning(const capaz_ctx* c, const char* key, int* value) {
     else if (k == "tile_y") *value = c->tuning.tile_y;
     else if (k == "kchunk") *value = c->tuning.kchunk;
     else if (k == "adjoint_kernel") *value = c->tuning.adjoint_kernel;
+    else if (k == "adjoint_streams") *value = c->tuning.adjoint_streams;
     else if (k == "energy_kernel") *value = c->tuning.energy_kernel;
     else if (k == "energy_newton") *value = c->tuning.energy_newton;
     else if (k == "timing") *value = c->timing ? 1 : 0;
@@ -552,7 +592,10 @@ static int face_call(capaz_ctx* c, int field, int axis, int side, double* buf, i
         return fail(c, CAPAZ_ERR_INVALID, "axis must be 0..2, side 0 or 1, buffer non-NULL");
     int n = axis == 0 ? c->p.Nx : (axis == 1 ? c->p.Ny : c->p.Nz);
     if (n < 2) return fail(c, CAPAZ_ERR_INVALID, "axis has fewer than 2 planes");
-    if ((rc = materialize(c))) return rc;
+    // a field of the ping-pong state is current as it is (its ghost planes do
+    // not enter the last step's intermediates, which come from the previous
+    // state); the others first need the last step's intermediates
+    if (!c->primary[field] && (rc = materialize(c))) return rc;
     // load_*: second / second-to-last plane; save_*: first / last plane
     int pos = save ? (side == 0 ? 0 : n - 1) : (side == 0 ? 1 : n - 2);
     CAPAZ_HIP(c, capaz::launch_face(c->p, c->cur[field], buf, axis, pos, save, c->stream));
@@ -566,6 +609,37 @@ int capaz_load_face(capaz_ctx* c, int field, int axis, int side, double* dbuf) {
 
 int capaz_save_face(capaz_ctx* c, int field, int axis, int side, const double* dbuf) {
     return face_call(c, field, axis, side, const_cast<double*>(dbuf), 1);
+}
+
+static int faces_call(capaz_ctx* c, const int* fields, int nfields, int axis, double* buf, int save) {
+    int rc;
+    if ((rc = check_ctx(c))) return rc;
+    if (!fields || !buf || nfields < 1 || nfields > capaz::kMaxFaceFields)
+        return fail(c, CAPAZ_ERR_INVALID, "fields and buffer non-NULL, 1 to 8 fields");
+    if (axis < 0 || axis > 2) return fail(c, CAPAZ_ERR_INVALID, "axis must be 0..2");
+    const int n = axis == 0 ? c->p.Nx : (axis == 1 ? c->p.Ny : c->p.Nz);
+    if (n < 2) return fail(c, CAPAZ_ERR_INVALID, "axis has fewer than 2 planes");
+    double* ptr[capaz::kMaxFaceFields];
+    bool all_primary = true;
+    for (int q = 0; q < nfields; ++q) {
+        if (!field_exists(c->user, fields[q])) return fail(c, CAPAZ_ERR_INVALID, "field does not exist for this model");
+        all_primary = all_primary && c->primary[fields[q]];
+    }
+    if (!all_primary && (rc = materialize(c))) return rc;
+    for (int q = 0; q < nfields; ++q) {
+        ptr[q] = c->cur[fields[q]];
+        if (save && fields[q] == capaz::F_m) c->m_checked = false;
+    }
+    CAPAZ_HIP(c, capaz::launch_faces(c->p, ptr, nfields, buf, axis, save, c->stream));
+    return CAPAZ_OK;
+}
+
+int capaz_load_faces(capaz_ctx* c, const int* fields, int nfields, int axis, double* dbuf) {
+    return faces_call(c, fields, nfields, axis, dbuf, 0);
+}
+
+int capaz_save_faces(capaz_ctx* c, const int* fields, int nfields, int axis, const double* dbuf) {
+    return faces_call(c, fields, nfields, axis, const_cast<double*>(dbuf), 1);
 }
 
 }  // extern "C"

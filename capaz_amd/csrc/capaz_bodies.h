@@ -44,6 +44,14 @@ CAPAZ_HD void body_P_Xi_ro(const DevParams& p, const DevFields& F, int64_t l) {
         relperm2(p, F.f[F_S_n][l], kw, kn, Pk);
         double Pn = Pw + Pk;
         F.f[F_P_n][l] = Pn;
+        if (p.energy) {  // the configurations' energy closure: mu(T), energy.cpp's ro(P, T)
+            const double T = F.f[F_T][l];
+            F.f[F_Xi_w][l] = xi_var(K, kw, mu_cfg(p, p.mu_w, T));
+            F.f[F_Xi_n][l] = xi_var(K, kn, mu_cfg(p, p.mu_n, T));
+            F.f[F_ro_w][l] = ro_w_T(p, Pw, T);
+            F.f[F_ro_n][l] = ro_n_T(p, Pn, T);
+            return;
+        }
         F.f[F_Xi_w][l] = xi_const(K, kw, p.mu_w, p.r_mu_w);
         F.f[F_Xi_n][l] = xi_const(K, kn, p.mu_n, p.r_mu_n);
         F.f[F_ro_w][l] = ro_w_of(p, Pw);
@@ -64,7 +72,13 @@ CAPAZ_HD void body_P_Xi_ro(const DevParams& p, const DevFields& F, int64_t l) {
     }
     if (p.energy) {
         double T = F.f[F_T][l], mw, mn, mg;
-        mu_of_T(T, mw, mn, mg);
+        if (p.ecfg) {
+            mw = mu_cfg(p, p.mu_w, T);
+            mn = mu_cfg(p, p.mu_n, T);
+            mg = mu_cfg(p, p.mu_g, T);
+        } else {
+            mu_of_T(T, mw, mn, mg);
+        }
         F.f[F_Xi_w][l] = xi_var(K, kw, mw);
         F.f[F_Xi_n][l] = xi_var(K, kn, mn);
         F.f[F_Xi_g][l] = xi_var(K, kg, mg);
@@ -166,28 +180,37 @@ CAPAZ_HD void body_roS(const DevParams& p, const DevFields& F, int64_t l) {
 // ---- energy: H_E_current_calculation (cpu.cpp:38-50; energy.cpp:132-143,317-325) ----
 CAPAZ_HD void body_HE(const DevParams& p, const DevFields& F, int64_t l) {
     double T = F.f[F_T][l];
-    double Hw = H_w_T(p, T), Hn = H_n_T(p, T), Hg = H_g_T(p, T), Hr = H_r_T(p, T);
+    double Hw = H_w_of(p, T), Hn = H_n_T(p, T), Hr = H_r_of(p, T);
     F.f[F_H_w][l] = Hw;
     F.f[F_H_n][l] = Hn;
-    F.f[F_H_g][l] = Hg;
     F.f[F_H_r][l] = Hr;
     double m = F.f[F_m][l];
+    if (p.model == 2) {  // assign_E_current without the gas term
+        F.f[F_E][l] = (m * (F.f[F_S_w][l] * (F.f[F_ro_w][l] * Hw - F.f[F_P_w][l]) +
+                            F.f[F_S_n][l] * (F.f[F_ro_n][l] * Hn - F.f[F_P_n][l])) +
+                       (1. - m) * (p.ro_r * Hr - F.f[F_P_w][l]));
+        return;
+    }
+    double Hg = H_g_T(p, T);
+    F.f[F_H_g][l] = Hg;
     F.f[F_E][l] = (m * (F.f[F_S_w][l] * (F.f[F_ro_w][l] * Hw - F.f[F_P_w][l]) +
                         F.f[F_S_n][l] * (F.f[F_ro_n][l] * Hn - F.f[F_P_n][l]) +
                         F.f[F_S_g][l] * (F.f[F_ro_g][l] * Hg - F.f[F_P_g][l])) +
-                   (1. - m) * (kRoR * Hr - F.f[F_P_w][l]));
+                   (1. - m) * (p.ro_r * Hr - F.f[F_P_w][l]));
 }
 
 // ---- energy: E_calculation -> assign_E_new (energy.cpp:235-340), interior cells ---
-CAPAZ_HD double lambda_eff_at(const DevFields& F, int64_t l) {
+CAPAZ_HD double lambda_eff_at(const DevParams& p, const DevFields& F, int64_t l) {
+    if (p.ecfg) return p.lam_eff;
     return lambda_eff(F.f[F_m][l], F.f[F_S_w][l], F.f[F_S_n][l], F.f[F_S_g][l], F.f[F_T][l]);
 }
 
 // lambda_eff of a cell, computed from the fields (the energy step's tiled E
 // pass reads the same values from an array instead).
 struct LamOfFields {
+    const DevParams& p;
     const DevFields& F;
-    CAPAZ_HD double operator()(int64_t l) const { return lambda_eff_at(F, l); }
+    CAPAZ_HD double operator()(int64_t l) const { return lambda_eff_at(p, F, l); }
 };
 
 template <class Lam>
@@ -196,12 +219,14 @@ CAPAZ_HD double conduction(const DevFields& F, const Lam& lam, int64_t l, int64_
     return exact_div(lam(l + s) * T[l + s] - 2 * lam(l) * T[l] + lam(l - s) * T[l - s], h2, rh2);
 }
 
-CAPAZ_HD double advection(const DevFields& F, const double* uw, const double* un, const double* ug, int64_t a,
-                          int64_t b) {
-    const double *rw = F.f[F_ro_w], *rn = F.f[F_ro_n], *rg = F.f[F_ro_g];
-    const double *Hw = F.f[F_H_w], *Hn = F.f[F_H_n], *Hg = F.f[F_H_g];
-    return rw[a] * Hw[a] * uw[a] - rw[b] * Hw[b] * uw[b] + rn[a] * Hn[a] * un[a] - rn[b] * Hn[b] * un[b] +
-           rg[a] * Hg[a] * ug[a] - rg[b] * Hg[b] * ug[b];
+CAPAZ_HD double advection(const DevParams& p, const DevFields& F, const double* uw, const double* un,
+                          const double* ug, int64_t a, int64_t b) {
+    const double *rw = F.f[F_ro_w], *rn = F.f[F_ro_n];
+    const double *Hw = F.f[F_H_w], *Hn = F.f[F_H_n];
+    const double s = rw[a] * Hw[a] * uw[a] - rw[b] * Hw[b] * uw[b] + rn[a] * Hn[a] * un[a] - rn[b] * Hn[b] * un[b];
+    if (p.model == 2) return s;  // no gas terms
+    const double *rg = F.f[F_ro_g], *Hg = F.f[F_H_g];
+    return s + rg[a] * Hg[a] * ug[a] - rg[b] * Hg[b] * ug[b];
 }
 
 template <class Lam>
@@ -212,21 +237,29 @@ CAPAZ_HD void body_E_with(const DevParams& p, const DevFields& F, const Lam& lam
     double T_flow = 0;
     if (p.Nx > 2) T_flow += conduction(F, lam, l, 1, p.hx2, p.r_hx2);
     if (p.Ny > 2) T_flow += conduction(F, lam, l, p.sy, p.hy2, p.r_hy2);
-    if (p.Nz > 2) T_flow = conduction(F, lam, l, p.sz, p.hz2, p.r_hz2);  // '=' as energy.cpp:257
+    if (p.Nz > 2) {
+        const double tz = conduction(F, lam, l, p.sz, p.hz2, p.r_hz2);
+        if (p.ecfg)
+            T_flow += tz;
+        else
+            T_flow = tz;  // '=' as energy.cpp:257
+    }
     double E_flow = 0;
     if (p.Nx > 2)
-        E_flow += exact_div(advection(F, F.f[F_ux_w], F.f[F_ux_n], F.f[F_ux_g], l + 1, l - 1), p.two_hx, p.r_two_hx);
+        E_flow += exact_div(advection(p, F, F.f[F_ux_w], F.f[F_ux_n], F.f[F_ux_g], l + 1, l - 1), p.two_hx,
+                            p.r_two_hx);
     if (p.Ny > 2)
-        E_flow += exact_div(advection(F, F.f[F_uy_w], F.f[F_uy_n], F.f[F_uy_g], l + p.sy, l - p.sy), p.two_hy,
+        E_flow += exact_div(advection(p, F, F.f[F_uy_w], F.f[F_uy_n], F.f[F_uy_g], l + p.sy, l - p.sy), p.two_hy,
                             p.r_two_hy);
-    if (p.Nz > 2)  // advects with uy, as energy.cpp:300-305
-        E_flow += exact_div(advection(F, F.f[F_uy_w], F.f[F_uy_n], F.f[F_uy_g], l + p.sz, l - p.sz), p.two_hz,
-                            p.r_two_hz);
+    if (p.Nz > 2) {  // the reference advects with uy here (energy.cpp:300-305)
+        const int w = p.ecfg ? F_uz_w : F_uy_w, n = p.ecfg ? F_uz_n : F_uy_n, g = p.ecfg ? F_uz_g : F_uy_g;
+        E_flow += exact_div(advection(p, F, F.f[w], F.f[n], F.f[g], l + p.sz, l - p.sz), p.two_hz, p.r_two_hz);
+    }
     double Q_hw = 0, Q_hr = 0;
     F.f[F_E_new][l] = F.f[F_E][l] + p.dt * (T_flow + Q_hw + Q_hr - E_flow);
 }
 
-CAPAZ_HD void body_E(const DevParams& p, const DevFields& F, int64_t l) { body_E_with(p, F, LamOfFields{F}, l); }
+CAPAZ_HD void body_E(const DevParams& p, const DevFields& F, int64_t l) { body_E_with(p, F, LamOfFields{p, F}, l); }
 
 // ---- P_S_calculation -> Newton (cpu.cpp:80-89), interior cells -------------------------
 // The energy model's Newton update of interior cell l without the pivot
@@ -253,9 +286,44 @@ CAPAZ_HD bool body_newton_energy_nopivot(const DevParams& p, const DevFields& F,
     return true;
 }
 
+// The two-phase energy model's Newton update of interior cell l without the
+// pivot search (false, nothing written: the cell needs the search).
+CAPAZ_HD bool body_newton_energy2_nopivot(const DevParams& p, const DevFields& F, int64_t l) {
+    double P = F.f[F_P_w][l], Sw = F.f[F_S_w][l], Sn = F.f[F_S_n][l], T = F.f[F_T][l];
+    double Hw = F.f[F_H_w][l], Hn = F.f[F_H_n][l], Hr = F.f[F_H_r][l];
+    const double E_new = F.f[F_E_new][l];
+    if (!newton_energy2_t<true>(p, Sw, Sn, P, T, Hw, Hn, Hr, F.f[F_m][l], F.f[F_roS_w][l], F.f[F_roS_n][l], E_new))
+        return false;
+    F.f[F_T][l] = T;
+    F.f[F_H_w][l] = Hw;
+    F.f[F_H_n][l] = Hn;
+    F.f[F_H_r][l] = Hr;
+    F.f[F_E][l] = E_new;
+    F.f[F_P_w][l] = P;
+    F.f[F_S_w][l] = Sw;
+    F.f[F_S_n][l] = Sn;
+    return true;
+}
+
 CAPAZ_HD void body_newton(const DevParams& p, const DevFields& F, int64_t l) {
     int i, j, k;
     decompose(p, l, i, j, k);
+    if (p.model == 2 && p.energy) {
+        if (!interior_newton2(p, i, j, k)) return;
+        double P = F.f[F_P_w][l], Sw = F.f[F_S_w][l], Sn = F.f[F_S_n][l], T = F.f[F_T][l];
+        double Hw = F.f[F_H_w][l], Hn = F.f[F_H_n][l], Hr = F.f[F_H_r][l];
+        const double E_new = F.f[F_E_new][l];
+        newton_energy2_t<false>(p, Sw, Sn, P, T, Hw, Hn, Hr, F.f[F_m][l], F.f[F_roS_w][l], F.f[F_roS_n][l], E_new);
+        F.f[F_T][l] = T;
+        F.f[F_H_w][l] = Hw;
+        F.f[F_H_n][l] = Hn;
+        F.f[F_H_r][l] = Hr;
+        F.f[F_E][l] = E_new;
+        F.f[F_P_w][l] = P;
+        F.f[F_S_w][l] = Sw;
+        F.f[F_S_n][l] = Sn;
+        return;
+    }
     if (p.model == 2) {
         if (!interior_newton2(p, i, j, k)) return;
         double P = F.f[F_P_w][l], Sn = F.f[F_S_n][l];
@@ -305,9 +373,12 @@ CAPAZ_HD void body_boundary(const DevParams& p, const DevFields& F, int64_t l) {
             Pw = P1;
         else if (j == 0)
             Pw = p.P_atm;
+        else if (p.P_bottom > 0)  // the configurations' fixed bottom pressure
+            Pw = p.P_bottom;
         else  // ro_eff_gdy (main.cpp:255-267)
             Pw = P1 + (F.f[F_ro_n][l1] * Sn1 + F.f[F_ro_w][l1] * (1 - Sn1)) * F.f[F_m][l1] * p.g * p.hy;
         F.f[F_P_w][l] = Pw;
+        if (p.energy) F.f[F_T][l] = border_T_cfg(p, j, F.f[F_T][l1]);
         return;
     }
     // three-phase.cpp:345-427
@@ -323,7 +394,9 @@ CAPAZ_HD void body_boundary(const DevParams& p, const DevFields& F, int64_t l) {
     F.f[F_P_w][l] = Pw;
     F.f[F_P_n][l] = Pn;
     F.f[F_P_g][l] = Pg;
-    if (p.energy) {  // Border_T (energy.cpp:343-366)
+    if (p.energy && p.ecfg) {
+        F.f[F_T][l] = border_T_cfg(p, j, F.f[F_T][l1]);
+    } else if (p.energy) {  // Border_T (energy.cpp:343-366)
         double T;
         if (j == 0)
             T = 400;
@@ -358,7 +431,7 @@ CAPAZ_HD void body_energy_a(const DevParams& p, const DevFields& F, double* lam,
     body_S(p, F, l);
     body_P_Xi_ro(p, F, l);
     body_HE(p, F, l);
-    lam[l] = lambda_eff_at(F, l);
+    lam[l] = lambda_eff_at(p, F, l);
 }
 
 // ---- fused step (two- and three-phase without energy) -------------------------------------
@@ -468,6 +541,8 @@ CAPAZ_HD void boundary_epilogue(const DevParams& p, const StepIO& io, int i, int
                         Pb = Pw;
                     else if (bj == 0)
                         Pb = p.P_atm;
+                    else if (p.P_bottom > 0)
+                        Pb = p.P_bottom;
                     else
                         Pb = Pw + ro_eff;
                     io.Pw1[lb] = Pb;

@@ -489,6 +489,21 @@ CAPAZ_HD double H_n_T(const DevParams& p, double T) { return p.H0_n + (T - kT0) 
 CAPAZ_HD double H_g_T(const DevParams& p, double T) { return p.H0_g + (T - kT0) * (kC0G + half(kCG * (T - kT0))); }
 CAPAZ_HD double H_r_T(const DevParams& p, double T) { return p.H0_r + (T - kT0) * (kC0R + half(kCR * (T - kT0))); }
 
+// ---- the configurations' energy closure (no reference counterpart) -----------------------
+// Selected by p.ecfg: a constant conductivity of the medium and rock heat
+// capacity, mu0 * exp(-a (T - T_ref)), and the water enthalpy as the integral
+// of c_w (energy.cpp:114's cubic term has the opposite sign).
+
+CAPAZ_HD double mu_cfg(const DevParams& p, double mu0, double T) { return mu0 * exp(-p.mu_T_a * (T - p.mu_T_ref)); }
+
+CAPAZ_HD double H_w_of(const DevParams& p, double T) {
+    if (p.ecfg)
+        return p.H0_w + (T - kT0) * (kC0W - (T - kT0) * (kCW * 0.5 - exact_div(kCW2 * (T - kT0), 3., 1. / 3.)));
+    return H_w_T(p, T);
+}
+CAPAZ_HD double H_r_of(const DevParams& p, double T) { return p.ecfg ? p.H0_r + (T - kT0) * p.c_r : H_r_T(p, T); }
+CAPAZ_HD double c_r_of(const DevParams& p, double T) { return p.ecfg ? p.c_r : c_r_T(T); }
+
 // mult_matrix_vector (gauss.cpp:94-104).
 template <int N>
 CAPAZ_HD void mult_matrix_vector(double* r, const double* m, const double* v) {
@@ -565,7 +580,7 @@ CAPAZ_HD bool newton_energy_t(const DevParams& p, double& Sw, double& Sn, double
         F[1] = rw * Sw - roS_w;
         F[2] = rn * Sn - roS_n;
         F[3] = rg * Sg - roS_g;
-        F[4] = m * (Sw * (rw * Hw - P) + Sn * (rn * Hn - P) + Sg * (rg * Hg - P)) + (1. - m) * (kRoR * Hr - P) - E_new;
+        F[4] = m * (Sw * (rw * Hw - P) + Sn * (rn * Hn - P) + Sg * (rg * Hg - P)) + (1. - m) * (p.ro_r * Hr - P) - E_new;
         dF[0] = 1.; dF[1] = 1.; dF[2] = 1.; dF[3] = 0.; dF[4] = 0.;
         dF[0 + n] = rw; dF[1 + n] = 0.; dF[2 + n] = 0.; dF[3 + n] = dwP * Sw; dF[4 + n] = dwT * Sw;
         dF[0 + n * 2] = 0.; dF[1 + n * 2] = rn; dF[2 + n * 2] = 0.; dF[3 + n * 2] = dnP * Sn; dF[4 + n * 2] = dnT * Sn;
@@ -576,7 +591,7 @@ CAPAZ_HD bool newton_energy_t(const DevParams& p, double& Sw, double& Sn, double
         dF[3 + n * 4] = m * (Sw * (dwP * Hw - 1.) + Sn * (dnP * Hn - 1.) + Sg * (dgP * Hg - 1.)) + (1. - m) * (-1);
         dF[4 + n * 4] = m * (Sw * (dwT * Hw + rw * c_w_T(T)) + Sn * (dnT * Hn + rn * c_n_T(T)) +
                              Sg * (dgT * Hg + rg * c_g_T(T))) +
-                        (1. - m) * kRoR * c_r_T(T);
+                        (1. - m) * p.ro_r * c_r_of(p, T);
         if constexpr (NOPIVOT) {
             if (!reverse_nopivot<5>(dF)) return false;
         } else {
@@ -588,10 +603,10 @@ CAPAZ_HD bool newton_energy_t(const DevParams& p, double& Sw, double& Sn, double
         Sg = Sg - corr[2];
         P = P - corr[3];
         T = T - corr[4];
-        Hw = H_w_T(p, T);
+        Hw = H_w_of(p, T);
         Hn = H_n_T(p, T);
         Hg = H_g_T(p, T);
-        Hr = H_r_T(p, T);
+        Hr = H_r_of(p, T);
     }
     return true;
 }
@@ -600,6 +615,48 @@ CAPAZ_HD void newton_energy(const DevParams& p, double& Sw, double& Sn, double& 
                             double& Hn, double& Hg, double& Hr, double m, double roS_w, double roS_n, double roS_g,
                             double E_new) {
     newton_energy_t<false>(p, Sw, Sn, Sg, P, T, Hw, Hn, Hg, Hr, m, roS_w, roS_n, roS_g, E_new);
+}
+
+// The two-phase energy model's Newton (configuration closure): energy.cpp:370-463's
+// system without the gas row and column, 4 x 4 in (S_w, S_n, P_w, T), the
+// densities at P_w for both phases as energy.cpp:388-394 has them, H_*
+// refreshed after every iteration.  NOPIVOT as newton_energy_t.
+template <bool NOPIVOT>
+CAPAZ_HD bool newton_energy2_t(const DevParams& p, double& Sw, double& Sn, double& P, double& T, double& Hw,
+                               double& Hn, double& Hr, double m, double roS_w, double roS_n, double E_new) {
+    constexpr int n = 4;
+    for (int w = 1; w <= p.iterations; w++) {
+        double rw = ro_w_T(p, P, T), rn = ro_n_T(p, P, T);
+        double dwP = p.ro0_w * p.beta_w, dwT = (-1) * p.ro0_w * kAlfaW;
+        double dnP = p.ro0_n * p.beta_n, dnT = (-1) * p.ro0_n * kAlfaN;
+        double F[4], dF[16], corr[4];
+        F[0] = Sw + Sn - 1.;
+        F[1] = rw * Sw - roS_w;
+        F[2] = rn * Sn - roS_n;
+        F[3] = m * (Sw * (rw * Hw - P) + Sn * (rn * Hn - P)) + (1. - m) * (p.ro_r * Hr - P) - E_new;
+        dF[0] = 1.; dF[1] = 1.; dF[2] = 0.; dF[3] = 0.;
+        dF[0 + n] = rw; dF[1 + n] = 0.; dF[2 + n] = dwP * Sw; dF[3 + n] = dwT * Sw;
+        dF[0 + n * 2] = 0.; dF[1 + n * 2] = rn; dF[2 + n * 2] = dnP * Sn; dF[3 + n * 2] = dnT * Sn;
+        dF[0 + n * 3] = m * (rw * Hw - P);
+        dF[1 + n * 3] = m * (rn * Hn - P);
+        dF[2 + n * 3] = m * (Sw * (dwP * Hw - 1.) + Sn * (dnP * Hn - 1.)) + (1. - m) * (-1);
+        dF[3 + n * 3] = m * (Sw * (dwT * Hw + rw * c_w_T(T)) + Sn * (dnT * Hn + rn * c_n_T(T))) +
+                        (1. - m) * p.ro_r * c_r_of(p, T);
+        if constexpr (NOPIVOT) {
+            if (!reverse_nopivot<4>(dF)) return false;
+        } else {
+            reverse_matrix<4>(dF);
+        }
+        mult_matrix_vector<4>(corr, dF, F);
+        Sw = Sw - corr[0];
+        Sn = Sn - corr[1];
+        P = P - corr[2];
+        T = T - corr[3];
+        Hw = H_w_of(p, T);
+        Hn = H_n_T(p, T);
+        Hr = H_r_of(p, T);
+    }
+    return true;
 }
 
 // ---- three-phase Border_P faces (three-phase.cpp:383-422) ---------------------------------
@@ -620,6 +677,15 @@ CAPAZ_HD void border_p3(const DevParams& p, int j, double P1, double& Pw, double
         Pn = exact_div(P1 + p.bp_n_num, p.bp_n_bot_den, p.r_bp_n_bot_den);
         Pg = exact_div(P1, p.bp_g_bot_den, p.r_bp_g_bot_den);
     }
+}
+
+// Border_T (energy.cpp:343-366) with the configurations' face temperatures:
+// top and bottom fixed where set (> 0), else, like the side faces, insulated
+// (the inward cell's T1).
+CAPAZ_HD double border_T_cfg(const DevParams& p, int j, double T1) {
+    if (j == 0 && p.T_top > 0) return p.T_top;
+    if (j == p.Ny - 1 && p.T_bottom > 0) return p.T_bottom;
+    return T1;
 }
 
 // Two-phase Border_S window test at the top face (two-phase.cpp:175-181).

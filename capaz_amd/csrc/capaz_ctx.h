@@ -37,6 +37,11 @@ struct capaz_ctx {
     double* adj_mem = nullptr;
     size_t adj_bytes = 0;
     long adj_segment = 0;  // segment length C of the last capaz_adjoint
+    // second stream of the reverse sweep (segment recompute beside the
+    // adjoint passes) and its events: [0..1] segment buffer b recomputed,
+    // [2..3] segment buffer b free again
+    hipStream_t adj_stream = nullptr;
+    hipEvent_t adj_ev[4] = {};
     std::string err;
 };
 

@@ -17,7 +17,12 @@ inline std::string validate(const capaz_params& u) {
     if (u.struct_size != sizeof(capaz_params)) return "capaz_params.struct_size does not match this library";
     if (u.model != 2 && u.model != 3) return "model must be CAPAZ_TWO_PHASE (2) or CAPAZ_THREE_PHASE (3)";
     if (u.energy != 0 && u.energy != 1) return "energy must be 0 or 1";
-    if (u.energy && u.model != 3) return "the energy equation exists for the three-phase model only (energy.cpp)";
+    if (u.energy_closure != 0 && u.energy_closure != 1)
+        return "energy_closure must be CAPAZ_ENERGY_REFERENCE or CAPAZ_ENERGY_CONFIG";
+    if (u.energy && u.model != 3 && u.energy_closure != 1)
+        return "the two-phase energy equation needs energy_closure CAPAZ_ENERGY_CONFIG (energy.cpp is three-phase)";
+    if (u.energy && u.energy_closure == 1 && !(u.lambda_eff > 0 && u.c_r > 0 && u.ro_r > 0))
+        return "CAPAZ_ENERGY_CONFIG needs lambda_eff, c_r and ro_r > 0";
     if (u.closure != 0 && u.closure != 1) return "closure must be CAPAZ_CLOSURE_REFERENCE or CAPAZ_CLOSURE_COREY";
     if (u.Ny < 3) return "Ny must be >= 3";
     if (!(u.Nx == 1 || u.Nx >= 3)) return "Nx must be 1 or >= 3";
@@ -29,7 +34,8 @@ inline std::string validate(const capaz_params& u) {
     if (!(u.dt > 0)) return "dt must be > 0";
     if (!(u.P_atm > 0)) return "P_atm must be > 0";
     if (!(u.mu_w > 0 && u.mu_n > 0)) return "mu_w and mu_n must be > 0";
-    if (u.model == 3 && !u.energy && !(u.mu_g > 0)) return "mu_g must be > 0";
+    if (u.model == 3 && (!u.energy || u.energy_closure == 1) && !(u.mu_g > 0)) return "mu_g must be > 0";
+    if (u.P_bottom < 0) return "P_bottom must be >= 0";
     if (u.closure == 1 && !(u.corey_n > 0)) return "corey_n must be > 0";
     if (u.model == 2 && u.closure == 0 && !(u.lambda != 0)) return "lambda must be non-zero";
     if (u.model == 3 && u.closure == 0 && !(u.lambda != 0 && u.lambda != 1)) return "lambda must not be 0 or 1";
@@ -136,7 +142,16 @@ inline DevParams derive(const capaz_params& u) {
     p.H0_w = u.P_atm / u.ro0_w;
     p.H0_n = u.P_atm / u.ro0_n;
     p.H0_g = u.ro0_g != 0 ? u.P_atm / u.ro0_g : 0.;
-    p.H0_r = u.P_atm / 2000.;
+    p.ecfg = u.energy && u.energy_closure == 1;
+    p.ro_r = p.ecfg ? u.ro_r : 2000.;
+    p.H0_r = u.P_atm / p.ro_r;
+    p.P_bottom = u.P_bottom;
+    p.lam_eff = u.lambda_eff;
+    p.c_r = u.c_r;
+    p.T_top = u.T_top;
+    p.T_bottom = u.T_bottom;
+    p.mu_T_a = u.mu_T_a;
+    p.mu_T_ref = u.mu_T_ref;
     return p;
 }
 

@@ -151,6 +151,8 @@ __device__ __forceinline__ void write_boundary(const DevParams& p, const StepIO&
             Pb = Pw;
         else if (bj == 0)
             Pb = p.P_atm;
+        else if (p.P_bottom > 0)  // the configurations' fixed bottom pressure
+            Pb = p.P_bottom;
         else
             Pb = Pw + ro_eff;
         io.Pw1[lb] = Pb;
@@ -462,14 +464,19 @@ __global__ void __launch_bounds__(TX * TY) k_roS_z(DevParams p, StepIO io, int k
 // E_new has its bits.
 // lam: lambda_eff per cell as the step's first pass stored it, or NULL (the
 // entry point) to evaluate it from m, S_w, S_n, S_g and T.
+// uzz: the velocities the z term advects with -- uy (energy.cpp:300-305), or
+// uz in the configurations' energy closure (p.ecfg), whose conductivity is
+// the constant p.lam_eff and whose z conduction adds to the x and y terms.
 struct EnergyEIO {
     const double *lam, *T, *E;
     const double *m, *Sw, *Sn, *Sg;
-    const double *ro[3], *H[3], *ux[3], *uy[3];
+    const double *ro[3], *H[3], *ux[3], *uy[3], *uzz[3];
     double* E_new;
+    double lam_cfg;  // > 0: the constant conductivity of p.ecfg
 };
 
 __device__ __forceinline__ double energy_lam(const EnergyEIO& a, int64_t l) {
+    if (a.lam_cfg > 0) return a.lam_cfg;
     return a.lam ? a.lam[l] : lambda_eff(a.m[l], a.Sw[l], a.Sn[l], a.Sg[l], a.T[l]);
 }
 
@@ -484,11 +491,11 @@ __device__ __forceinline__ void energy_e_vals(const EnergyEIO& a, int64_t l, dou
     }
 }
 
-// lambda_eff * T and the uy products of a z neighbour: w[0], w[1 + a]
+// lambda_eff * T and the uzz products of a z neighbour: w[0], w[1 + a]
 __device__ __forceinline__ void energy_e_zvals(const EnergyEIO& a, int64_t l, double* w) {
     w[0] = energy_lam(a, l) * a.T[l];
 #pragma unroll
-    for (int ph = 0; ph < 3; ++ph) w[1 + ph] = a.ro[ph][l] * a.H[ph][l] * a.uy[ph][l];
+    for (int ph = 0; ph < 3; ++ph) w[1 + ph] = a.ro[ph][l] * a.H[ph][l] * a.uzz[ph][l];
 }
 
 // advection's (((X_w+ - X_w-) + X_n+) - X_n-) + X_g+) - X_g- (the body's order)
@@ -547,7 +554,13 @@ __global__ void __launch_bounds__(TX * TY) k_energy_e_z(DevParams p, EnergyEIO a
                 T_flow += exact_div(lds[0][ty + 1][tx + 2] - 2 * lc * Tc + lds[0][ty + 1][tx], p.hx2, p.r_hx2);
             if (p.Ny > 2)
                 T_flow += exact_div(lds[0][ty + 2][tx + 1] - 2 * lc * Tc + lds[0][ty][tx + 1], p.hy2, p.r_hy2);
-            if (p.Nz > 2) T_flow = exact_div(zp[0] - 2 * lc * Tc + zm[0], p.hz2, p.r_hz2);  // '=' as energy.cpp:257
+            if (p.Nz > 2) {
+                const double tz = exact_div(zp[0] - 2 * lc * Tc + zm[0], p.hz2, p.r_hz2);
+                if (p.ecfg)
+                    T_flow += tz;
+                else
+                    T_flow = tz;  // '=' as energy.cpp:257
+            }
             double E_flow = 0;
             if (p.Nx > 2)
                 E_flow += exact_div(energy_adv(lds[1][ty + 1][tx + 2], lds[1][ty + 1][tx], lds[2][ty + 1][tx + 2],
@@ -563,10 +576,14 @@ __global__ void __launch_bounds__(TX * TY) k_energy_e_z(DevParams p, EnergyEIO a
             a.E_new[l] = a.E[l] + p.dt * (T_flow + Q_hw + Q_hr - E_flow);
         }
         __syncthreads();
-        // roll the z window: the centre's lambda_eff * T and uy products
-        zm[0] = v[0];
+        // roll the z window: the centre's lambda_eff * T and uzz products
+        if (p.ecfg) {
+            if (own_in_grid) energy_e_zvals(a, l, zm);
+        } else {
+            zm[0] = v[0];
 #pragma unroll
-        for (int ph = 0; ph < 3; ++ph) zm[1 + ph] = v[4 + ph];
+            for (int ph = 0; ph < 3; ++ph) zm[1 + ph] = v[4 + ph];
+        }
     }
 }
 

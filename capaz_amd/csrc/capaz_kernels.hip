@@ -70,7 +70,10 @@ __global__ void __launch_bounds__(kBlock) k_newton_energy(DevParams p, DevFields
     if (l >= p.n) return;
     int i, j, k;
     decompose(p, l, i, j, k);
-    redo[l] = interior3(p, i, j, k) && !body_newton_energy_nopivot(p, F, l) ? 1 : 0;
+    if (p.model == 2)
+        redo[l] = interior_newton2(p, i, j, k) && !body_newton_energy2_nopivot(p, F, l) ? 1 : 0;
+    else
+        redo[l] = interior3(p, i, j, k) && !body_newton_energy_nopivot(p, F, l) ? 1 : 0;
 }
 // One thread per 16 marks (one 16-byte load; redo is allocated rounded up to
 // 16 bytes, the marks past Nx*Ny*Nz are never read as cells).
@@ -138,6 +141,55 @@ __global__ void __launch_bounds__(kBlock) k_face(DevParams p, double* field, dou
         field[l] = buf[t];
     else
         buf[t] = field[l];
+}
+
+// All the exchange planes of one halo exchange in one launch.  Grid row y
+// is side s of field q (y = q * 2 + s, the buffer layout: that side's plane
+// at buf + y * plane, cell c = a + na * b as k_face numbers it), so the
+// field pointer and the plane position are uniform over a workgroup.  A z
+// plane is contiguous in the field too (sy == Nx); with vec set (checked on
+// the host: even plane, 16-byte aligned field and buffer) each lane moves two
+// doubles of it.
+struct FaceFields {
+    double* f[kMaxFaceFields];
+};
+__global__ void __launch_bounds__(kBlock) k_faces(DevParams p, FaceFields ff, double* buf, int axis, int save,
+                                                  int vec) {
+    const int y = (int)blockIdx.y, q = y >> 1, s = y & 1;
+    const uint32_t na = (uint32_t)(axis == 0 ? p.Ny : p.Nx), nb = (uint32_t)(axis == 2 ? p.Ny : p.Nz);
+    const uint32_t plane = na * nb;
+    const int n = axis == 0 ? p.Nx : (axis == 1 ? p.Ny : p.Nz);
+    // load: second / second-to-last plane; save: first / last plane
+    const int pos = save ? (s == 0 ? 0 : n - 1) : (s == 0 ? 1 : n - 2);
+    double* const f = ff.f[q];
+    double* const side = buf + (int64_t)y * plane;
+    const uint32_t t = blockIdx.x * (uint32_t)kBlock + threadIdx.x;
+    if (axis == 2 && p.sy == p.Nx) {
+        double* const fp = f + (int64_t)pos * p.sz;
+        if (vec) {
+            const uint32_t c = 2 * t;
+            if (c >= plane) return;
+            double2* dst = reinterpret_cast<double2*>(save ? fp + c : side + c);
+            const double2* src = reinterpret_cast<const double2*>(save ? side + c : fp + c);
+            *dst = *src;
+        } else if (t < plane) {
+            if (save)
+                fp[t] = side[t];
+            else
+                side[t] = fp[t];
+        }
+        return;
+    }
+    if (t >= plane) return;
+    const uint32_t a = t % na, b = t / na;
+    int64_t l;
+    if (axis == 0) l = (int64_t)pos + (int64_t)a * p.sy + (int64_t)b * p.sz;
+    else if (axis == 1) l = (int64_t)a + (int64_t)pos * p.sy + (int64_t)b * p.sz;
+    else l = (int64_t)a + (int64_t)b * p.sy + (int64_t)pos * p.sz;
+    if (save)
+        f[l] = side[t];
+    else
+        side[t] = f[l];
 }
 
 // Non-finite sentinel: the smallest cell index at which one of up to four
@@ -229,11 +281,12 @@ int fused_kchunk(const DevParams& p, int kchunk, int ty, int resident) {
 template <int MODEL, int CLOSURE, int TY, bool CARRY>
 hipError_t launch_z(const DevParams& p, const StepIO& io, int kchunk, hipStream_t s, const KernelEvents* ev) {
     const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
-    kchunk = fused_kchunk(p, kchunk, TY, resident_groups<k_fused_z<MODEL, CLOSURE, 64, TY, CARRY>>(64 * TY));
+    constexpr auto kernel = k_fused_z<MODEL, CLOSURE, 64, TY, CARRY>;
+    kchunk = fused_kchunk(p, kchunk, TY, resident_groups<kernel>(64 * TY));
     dim3 grid((nx + 63) / 64, (ny + TY - 1) / TY, (nz + kchunk - 1) / kchunk);
     hipError_t e;
     if (ev && (e = hipEventRecord(ev->begin, s)) != hipSuccess) return e;
-    hipLaunchKernelGGL((k_fused_z<MODEL, CLOSURE, 64, TY, CARRY>), grid, dim3(64, TY), 0, s, p, io, kchunk);
+    hipLaunchKernelGGL(kernel, grid, dim3(64, TY), 0, s, p, io, kchunk);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev->end, s)) != hipSuccess) return e;
     if constexpr (!kFusedFaces<MODEL>)
@@ -335,13 +388,15 @@ hipError_t launch_energy_e_tiles(const DevParams& p, const DevFields& F, const d
     ea.Sn = F.f[F_S_n];
     ea.Sg = F.f[F_S_g];
     const int ro[3] = {F_ro_w, F_ro_n, F_ro_g}, H[3] = {F_H_w, F_H_n, F_H_g};
-    const int ux[3] = {F_ux_w, F_ux_n, F_ux_g}, uy[3] = {F_uy_w, F_uy_n, F_uy_g};
+    const int ux[3] = {F_ux_w, F_ux_n, F_ux_g}, uy[3] = {F_uy_w, F_uy_n, F_uy_g}, uz[3] = {F_uz_w, F_uz_n, F_uz_g};
     for (int a = 0; a < 3; ++a) {
         ea.ro[a] = F.f[ro[a]];
         ea.H[a] = F.f[H[a]];
         ea.ux[a] = F.f[ux[a]];
         ea.uy[a] = F.f[uy[a]];
+        ea.uzz[a] = F.f[p.ecfg ? uz[a] : uy[a]];
     }
+    ea.lam_cfg = p.ecfg ? p.lam_eff : 0.;
     const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
     const int kc = fused_kchunk(p, 0, 8, resident_groups<k_energy_e_z<64, 8>>(512));
     dim3 ge((nx + 63) / 64, (ny + 7) / 8, (nz + kc - 1) / kc);
@@ -396,6 +451,24 @@ hipError_t launch_find_nonfinite(const DevParams& p, const double* const* f, uns
                                  hipStream_t s) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(kBlock), 0, s, first, ~0ull);
     hipLaunchKernelGGL(k_nonfinite, grid_for(p.n), dim3(kBlock), 0, s, p, f[0], f[1], f[2], f[3], first);
+    return hipGetLastError();
+}
+
+hipError_t launch_faces(const DevParams& p, double* const* fields, int nfields, double* buf, int axis, int save,
+                        hipStream_t s) {
+    if (nfields < 1 || nfields > kMaxFaceFields) return hipErrorInvalidValue;
+    const int64_t na = axis == 0 ? p.Ny : p.Nx, nb = axis == 2 ? p.Ny : p.Nz, plane = na * nb;
+    if (plane * 2 * nfields >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
+    FaceFields ff{};
+    bool aligned = ((uintptr_t)buf & 15) == 0;
+    for (int q = 0; q < nfields; ++q) {
+        ff.f[q] = fields[q];
+        aligned = aligned && ((uintptr_t)fields[q] & 15) == 0;
+    }
+    const int vec = axis == 2 && p.sy == p.Nx && plane % 2 == 0 && p.sz % 2 == 0 && aligned;
+    const int64_t lanes = vec ? plane / 2 : plane;
+    const dim3 grid((unsigned)((lanes + kBlock - 1) / kBlock), (unsigned)(2 * nfields));
+    hipLaunchKernelGGL(k_faces, grid, dim3(kBlock), 0, s, p, ff, buf, axis, save, vec);
     return hipGetLastError();
 }
 

@@ -18,9 +18,13 @@ enum class Entry { S, P_Xi_ro, U, RoS, HE, E, Newton, Boundary };
 // recorded by the segment's forward recompute, then the coefficient pass per
 // cell; 1 = both passes on the z-marching LDS tiles; 0 = both per cell,
 // recomputing the neighbours' coefficients.
-// energy_kernel: 1 = the energy model's step in six passes (S, P_Xi_ro, HE
-// and lambda_eff per cell; boundary u; roS and interior u on z tiles; E on z
-// tiles from the stored lambda_eff; Newton; boundary faces), 0 = the eight
+// adjoint_streams: 2 = the reverse sweep recomputes segment q-1 on a second
+// stream while the adjoint passes run over segment q (two segment buffers),
+// 1 = one stream, recompute then passes.
+// energy_kernel: 1 = the three-phase energy model's step in six passes (S,
+// P_Xi_ro, HE and lambda_eff per cell; boundary u; roS and interior u on z
+// tiles; E on z tiles from the stored lambda_eff; Newton; boundary faces),
+// the two-phase energy model's in one fused kernel; 0 = the eight
 // entry-point kernels.
 // energy_newton: 1 = the energy model's Newton in two passes (every interior
 // cell without the pivot search, then the cells that need it), 0 = the
@@ -31,6 +35,7 @@ struct Tuning {
     int tile_y = 8;
     int kchunk = 0;
     int adjoint_kernel = 2;
+    int adjoint_streams = 2;
     int energy_kernel = 1;
     int energy_newton = 1;
 };
@@ -64,6 +69,11 @@ hipError_t launch_energy_step(const DevParams& p, const DevFields& F, double* la
 // up to 16) with the general routine; redo NULL: the general routine on every
 // cell in one pass.
 hipError_t launch_newton_energy(const DevParams& p, const DevFields& F, unsigned char* redo, hipStream_t s);
+// One fused step of the two-phase energy model (configuration closure):
+// cur = {P_w, S_n, T, m, K}, next = {P_w, S_n, T} (every cell written);
+// kchunk as for the fused step; ev brackets the kernel.
+hipError_t launch_energy2_step(const DevParams& p, const double* const* cur, double* const* next, int kchunk,
+                               hipStream_t s, const KernelEvents* ev = nullptr);
 // Planes per tile of the fused z kernels: kchunk if >= 1, else chosen from
 // the grid and the number of workgroups resident at once, so that small grids
 // fill the GPU without a nearly empty last round.
@@ -76,6 +86,11 @@ hipError_t launch_dtm(const DevParams& p, const double* m, double* dtm, hipStrea
 hipError_t launch_copy_interior(const DevParams& p, const double* src, double* dst, hipStream_t s);
 hipError_t launch_m_zero(const DevParams& p, const double* m, int* flag, hipStream_t s);
 hipError_t launch_face(const DevParams& p, double* field, double* buf, int axis, int pos, int save, hipStream_t s);
+// Both exchange planes of nfields (<= kMaxFaceFields) fields along `axis` in
+// one launch, buffer layout [field][side][plane].
+constexpr int kMaxFaceFields = 8;
+hipError_t launch_faces(const DevParams& p, double* const* fields, int nfields, double* buf, int axis, int save,
+                        hipStream_t s);
 // *first = the smallest cell index at which f[0], f[1] or (where not NULL)
 // f[2], f[3] is not finite, ~0 if there is none.
 hipError_t launch_find_nonfinite(const DevParams& p, const double* const* f, unsigned long long* first,

@@ -45,8 +45,17 @@ struct DevParams {
     double bp_w_num, bp_w_top_den, bp_w_bot_den, r_bp_w_top_den, r_bp_w_bot_den;
     double bp_n_num, bp_n_top_den, bp_n_bot_den, r_bp_n_top_den, r_bp_n_bot_den;
     double bp_g_top_den, bp_g_bot_den, r_bp_g_top_den, r_bp_g_bot_den;
-    // energy: P_atm/ro0_* (energy.cpp:114-129)
+    // energy: P_atm/ro0_* and P_atm/ro_r (energy.cpp:114-129)
     double H0_w, H0_n, H0_g, H0_r;
+
+    // configuration closures (no reference counterpart): the two-phase fixed
+    // bottom pressure (> 0) and the energy closure (ecfg = 1): constant
+    // conductivity, rock heat capacity and density, face temperatures
+    // (<= 0: insulated) and mu(T) = mu0 exp(-mu_T_a (T - mu_T_ref)).  In the
+    // reference energy closure c_r and lam_eff are unused and ro_r = 2000.
+    double P_bottom;
+    int ecfg;
+    double lam_eff, c_r, ro_r, T_top, T_bottom, mu_T_a, mu_T_ref;
 };
 
 }  // namespace capaz

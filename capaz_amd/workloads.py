@@ -1,10 +1,11 @@
 """Seeded synthetic inputs of the benchmark workloads.
 
 Each builder returns ``(keys, fields)``: a deck of defines.ini keys
-(main.cpp:1327-1626) plus the backend's MODEL / ENERGY / CLOSURE / COREY_N
-keys, and the initial primary fields as (Nz, Ny, Nx) float64 arrays, x
+(main.cpp:1327-1626) plus the backend's own keys (MODEL, ENERGY, CLOSURE,
+COREY_N, P_BOTTOM and the energy closure's), and the initial primary fields as (Nz, Ny, Nx) float64 arrays, x
 fastest.  Random fields come from numpy's PCG64 (two-phase Buckley-Leverett
-seed 1000, three-phase injection seed 1001).  "Top" is j = 0 (gravity along
+seed 1000, three-phase injection seed 1001, non-isothermal two-phase seed
+1002).  "Top" is j = 0 (gravity along
 +j, cpu.cpp:392).
 
 Parameter substitutions (the decks as specified are unstable for this
@@ -32,20 +33,44 @@ def _porosity_noise(rng, shape):
 
 def buckley_leverett_2ph(n: int = 40):
     """Two-phase water/NAPL, n^3, Corey-2 (S_wr 0.1, S_nr 0.05), scalar K,
-    S_w0 = 0.15, top face injects S_w = 0.9 at P = 1.2e5 Pa.
+    S_w0 = 0.15, top face injects S_w = 0.9 at P = 1.2e5 Pa, bottom face
+    held at P = 1e5 Pa, side faces no-flow.
 
-    The reference's two-phase boundary sets the top pressure to P_atm and the
-    bottom by a hydrostatic increment (two-phase.cpp:200-211), so the top
-    pressure is carried by P_ATM = 1.2e5 and the bottom is the reference's
-    hydrostatic rule, not a fixed 1e5 Pa."""
+    The reference's two-phase boundary sets the top pressure to P_atm
+    (two-phase.cpp:200-211), so the top pressure is carried by
+    P_ATM = 1.2e5; the bottom takes the backend's fixed bottom pressure
+    (P_BOTTOM) instead of the reference's hydrostatic increment."""
     rng = np.random.default_rng(1000)
     shape = (n, n, n)
     keys = dict(MODEL=2, CLOSURE="corey", COREY_N=2, NX=n, NY=n, NZ=n, HX=0.01, HY=0.01, HZ=0.01, DT=1e-4,
-                P_ATM=1.2e5, G_CONST=9.807, MU_W=1e-3, MU_N=5e-3, RO_W=1000.0, RO_N=850.0,
+                P_ATM=1.2e5, P_BOTTOM=1.0e5, G_CONST=9.807, MU_W=1e-3, MU_N=5e-3, RO_W=1000.0, RO_N=850.0,
                 BETA_W=1e-6, BETA_N=1e-6, S_WR_0=0.1, S_NR_0=0.05, SOURCE=n, S_N_GR=0.1,
                 ITERATIONS=4, K_0=K_MEAN)
     P = np.full(shape, keys["P_ATM"])
     fields = dict(P_w=P, S_n=np.full(shape, 0.85), m=_porosity_noise(rng, shape), K=np.full(shape, K_MEAN))
+    return keys, fields
+
+
+def nonisothermal_2ph(nx: int = 256, ny: int = 256, nz: int = 128, seed: int = 1002):
+    """Non-isothermal two-phase water/NAPL on an nx x ny x nz grid (256 x 256 x
+    128 as specified), 1000 steps: the Buckley-Leverett physics and initial
+    state of buckley_leverett_2ph (Corey-2, S_wr 0.1, S_nr 0.05, scalar K
+    6.64e-11, porosity 0.4(1 + 0.05 N(0,1)) clipped to [0.3, 0.5], S_w0 0.15,
+    the whole top face injecting S_w 0.9 at P 1.2e5 Pa, bottom at 1e5 Pa,
+    no-flow sides, beta 1e-6 1/Pa, dt 1e-4 s, 4 Newton iterations) with the
+    energy equation: rock heat capacity 800 J/(kg K) and density 2000 kg/m^3,
+    thermal conductivity 1.5 W/(m K), T0 = 293 K, inflow at 353 K on the top
+    face (the other faces insulated), mu_a(T) = mu_a exp(-0.02 (T - 293)).
+    The fluids' enthalpies, heat capacities and thermal expansion are
+    energy.cpp's (T_0 = 273 K their base temperature)."""
+    keys, _ = buckley_leverett_2ph(4)
+    keys = dict(keys, NX=nx, NY=ny, NZ=nz, SOURCE=max(nx, nz), ENERGY=1, ENERGY_CLOSURE="config",
+                LAMBDA_EFF=1.5, C_R=800.0, RO_R=2000.0, T_TOP=353.0, T_BOTTOM=0.0, MU_T_A=0.02,
+                MU_T_REF=293.0)
+    rng = np.random.default_rng(seed)
+    shape = (nz, ny, nx)
+    fields = dict(P_w=np.full(shape, keys["P_ATM"]), S_n=np.full(shape, 0.85), T=np.full(shape, 293.0),
+                  m=_porosity_noise(rng, shape), K=np.full(shape, K_MEAN))
     return keys, fields
 
 

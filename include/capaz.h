@@ -60,6 +60,21 @@ typedef enum capaz_closure {
     CAPAZ_CLOSURE_COREY = 1
 } capaz_closure;
 
+typedef enum capaz_energy_closure {
+    /* energy.cpp's constants and closures, with its two z-term quirks
+     * (energy.cpp:257, 300-305); three phases only. */
+    CAPAZ_ENERGY_REFERENCE = 0,
+    /* The configurations' closure, for two or three phases: energy.cpp's
+     * fluid densities, heat capacities and enthalpies (the water enthalpy
+     * as the integral of its c_w), a constant conductivity lambda_eff and
+     * rock heat capacity c_r, mu_a(T) = mu_a exp(-mu_T_a (T - mu_T_ref)),
+     * fixed top/bottom temperatures, the conduction summed over the three
+     * axes and the z advection with u_z.  The two-phase energy model's
+     * Newton is energy.cpp:370-463's system without the gas row and
+     * column (4 x 4 in S_w, S_n, P_w, T).  No reference counterpart. */
+    CAPAZ_ENERGY_CONFIG = 1
+} capaz_energy_closure;
+
 /* Field identifiers, in the member order of ptr_Arrays (defines.h:65-76). */
 typedef enum capaz_field_id {
     CAPAZ_S_n = 0, CAPAZ_S_w, CAPAZ_P_w, CAPAZ_P_n, CAPAZ_ro_w, CAPAZ_ro_n,
@@ -78,7 +93,8 @@ typedef enum capaz_field_id {
 typedef struct capaz_params {
     unsigned struct_size;  /* sizeof(capaz_params), for ABI checks */
     int model;             /* capaz_model */
-    int energy;            /* 1: energy equation (three-phase only, as energy.cpp) */
+    int energy;            /* 1: energy equation (energy.cpp; two phases with
+                              CAPAZ_ENERGY_CONFIG only) */
     int closure;           /* capaz_closure */
     int Nx, Ny, Nz;        /* [NX] [NY] [NZ] */
     int source;            /* [SOURCE] top-face injection half-width (>0: inject) */
@@ -94,6 +110,17 @@ typedef struct capaz_params {
     double P_d;                 /* [P_D_0] two-phase entry pressure */
     double S_w_gr, S_n_gr;      /* [S_W_GR] [S_N_GR] injected saturations */
     double corey_n;             /* Corey exponent (CAPAZ_CLOSURE_COREY) */
+    /* Boundary and energy closures of the benchmark configurations, no
+     * reference counterpart (zero selects the reference's rule): */
+    double P_bottom;            /* two phases: > 0 a fixed bottom-face pressure
+                                   instead of the hydrostatic rule of
+                                   two-phase.cpp:100-107 */
+    int energy_closure;         /* capaz_energy_closure */
+    double lambda_eff;          /* CONFIG: thermal conductivity of the medium, W/(m K) */
+    double c_r, ro_r;           /* CONFIG: rock heat capacity J/(kg K), density kg/m^3 */
+    double T_top, T_bottom;     /* CONFIG: top/bottom face temperatures, K;
+                                   <= 0: insulated (copied from the inward cell) */
+    double mu_T_a, mu_T_ref;    /* CONFIG: mu_a(T) = mu_a * exp(-mu_T_a * (T - mu_T_ref)) */
 } capaz_params;
 
 /* Fills *p with zeros, struct_size and defaults (newton_iterations = 1,
@@ -169,10 +196,17 @@ int capaz_step(capaz_ctx* ctx, long first_step, long nsteps);
  *                   Newton from the flux divergence recorded while each
  *                   segment is recomputed; 1 both passes on z-marching LDS
  *                   tiles; 0 both one thread per cell
- *   "energy_kernel" the energy model's step: 1 (default) six passes (per
- *                   cell S, P_Xi_ro, HE and the conductivity; u; roS; E
- *                   from the stored conductivity; Newton; the boundary
- *                   faces); 0 the eight entry-point kernels
+ *   "adjoint_streams" 2 (default): capaz_adjoint recomputes the next
+ *                   segment on a second stream while the adjoint passes
+ *                   run over the current one (two segment buffers); 1: one
+ *                   stream
+ *   "energy_kernel" the energy model's step: 1 (default) for three phases
+ *                   six passes (per cell S, P_Xi_ro, HE and the
+ *                   conductivity; u; roS; E from the stored conductivity;
+ *                   Newton; the boundary faces), for two phases one fused
+ *                   kernel on z-marching 64 x 8 tiles (P_w, S_n, T read
+ *                   once and written once per step, the Newton in
+ *                   registers); 0 the eight entry-point kernels
  *   "energy_newton" the energy model's Newton (its entry point and step):
  *                   1 (default) every interior cell without the pivot
  *                   search, then the cells that need it with the search;
@@ -180,9 +214,9 @@ int capaz_step(capaz_ctx* ctx, long first_step, long nsteps);
  *   "timing"        1: record HIP events around each fused step's main
  *                   kernel (the energy model's whole step; read with
  *                   capaz_kernel_time); 0: off (default).
+ *                   Setting it resets the totals.
  *   "adjoint_segment" (read only) the segment length of the last
  *                   capaz_adjoint
- *                   Setting it resets the totals.
  *   "check_every"   K > 0: capaz_step runs capaz_check_finite after every
  *                   step whose number is a multiple of K and stops with
  *                   CAPAZ_ERR_NONFINITE at the first failing check (the
@@ -237,6 +271,16 @@ int capaz_stream(capaz_ctx* ctx, void** stream);
  * plane's size; the plane is laid out as the reference's HostBuffer. */
 int capaz_load_face(capaz_ctx* ctx, int field, int axis, int side, double* dbuf);
 int capaz_save_face(capaz_ctx* ctx, int field, int axis, int side, const double* dbuf);
+
+/* One halo exchange's planes in one launch: the per-field, per-side
+ * load_/save_exchange_data_part_* calls of an exchange (gpu.cu:854-1128,
+ * one kernel per face there) as a single kernel.  capaz_load_faces packs,
+ * for each of the nfields (1..8) fields, its second and second-to-last plane
+ * normal to `axis` into dbuf as [field q][side 0, 1][plane] (each plane laid
+ * out as capaz_load_face's); capaz_save_faces unpacks such a buffer into the
+ * first and last planes.  dbuf: device memory of 2 * nfields planes. */
+int capaz_load_faces(capaz_ctx* ctx, const int* fields, int nfields, int axis, double* dbuf);
+int capaz_save_faces(capaz_ctx* ctx, const int* fields, int nfields, int axis, const double* dbuf);
 
 /* ---- Checkpoint / restart ---------------------------------------------------
  * Replace save / restore (main.cpp:1035-1145) with a format that keeps every

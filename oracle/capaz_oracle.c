@@ -16,7 +16,14 @@
  * Pinning: the whole restatement is compared bit for bit with the compiled
  * reference (oracle/_ref) in the reference-closure mode by tests/test_oracle.py
  * and with the committed golden vectors by tests/test_golden.py.  The "corey"
- * closure has no reference counterpart; its k_r are pinned by closed forms. */
+ * closure has no reference counterpart; its k_r are pinned by closed forms.
+ * Neither have the configurations' energy closure (energy_closure 1: the
+ * two-phase energy model, constant conductivity and rock heat capacity,
+ * exponential mu(T), fixed face temperatures, the z terms of energy.cpp
+ * without its quirks) and the fixed bottom pressure; tests/test_oracle_config.py
+ * pins them by closed forms and invariants (stationary uniform states, the
+ * discrete Laplacian and advection of polynomial fields, energy conservation
+ * over insulated faces, the converged Newton residual). */
 #include "capaz_oracle.h"
 
 #include <math.h>
@@ -166,24 +173,44 @@ static double lambda_n(double T) { return lambda0_n * (1 - lambdaA_n * (T - T_0)
 static double lambda_g(double T) { return lambda0_g * pow((T / T_0), lambdaA_g); }
 static double lambda_r(double T) { (void)T; return lambda0_r; }
 
-static double lambda_eff(const oracle_fields* f, size_t l) {
+/* The configurations' energy closure (energy_closure 1; no reference
+ * counterpart): energy.cpp's fluid properties above (H_w, H_n, H_g, c_w, c_n,
+ * c_g, alfa, T_0) with a constant thermal conductivity of the medium, a
+ * constant rock heat capacity and density, mu_a(T) = mu_a exp(-a (T - T_ref))
+ * and prescribed face temperatures; the two-phase energy model exists only
+ * with it. */
+static int cfg(const oracle_params* p) { return p->energy_closure == 1 || p->model == 2; }
+static double rock_ro(const oracle_params* p) { return cfg(p) ? p->ro_r : ro_r; }
+static double rock_c(const oracle_params* p, double T) { return cfg(p) ? p->c_r : c_r(T); }
+static double mu_of_T(const oracle_params* p, double mu0, double T) {
+    return mu0 * exp(-p->mu_T_a * (T - p->mu_T_ref));
+}
+
+static double lambda_eff(const oracle_params* p, const oracle_fields* f, size_t l) {
+    if (cfg(p)) return p->lambda_eff;
     double T = f->T[l];
     return f->m[l] * (f->S_w[l] * lambda_w(T) + f->S_n[l] * lambda_n(T) + f->S_g[l] * lambda_g(T)) +
            (1. - f->m[l]) * lambda_r(T);
 }
 
 static double H_w_of(const oracle_params* p, double T) {
+    /* energy.cpp:114; its cubic term has the opposite sign of the integral of
+     * c_w (energy.cpp:34), which the configuration closure takes instead */
+    if (cfg(p)) return (p->P_atm / p->ro0_w) + (T - T_0) * (c0_w - (T - T_0) * (C_w / 2 - C_w2 * (T - T_0) / 3));
     return (p->P_atm / p->ro0_w) + (T - T_0) * (c0_w - (T - T_0) * (C_w / 2 + C_w2 * (T - T_0) / 3));
 }
 static double H_n_of(const oracle_params* p, double T) { return (p->P_atm / p->ro0_n) + (T - T_0) * (c0_n + C_n * (T - T_0) / 2); }
 static double H_g_of(const oracle_params* p, double T) { return (p->P_atm / p->ro0_g) + (T - T_0) * (c0_g + C_g * (T - T_0) / 2); }
-static double H_r_of(const oracle_params* p, double T) { return (p->P_atm / ro_r) + (T - T_0) * (c0_r + C_r * (T - T_0) / 2); }
+static double H_r_of(const oracle_params* p, double T) {
+    if (cfg(p)) return (p->P_atm / p->ro_r) + (T - T_0) * p->c_r; /* the integral of a constant c_r */
+    return (p->P_atm / ro_r) + (T - T_0) * (c0_r + C_r * (T - T_0) / 2);
+}
 
 static void assign_H(const oracle_params* p, oracle_fields* f, size_t l) {
     double T = f->T[l];
     f->H_w[l] = H_w_of(p, T);
     f->H_n[l] = H_n_of(p, T);
-    f->H_g[l] = H_g_of(p, T);
+    if (p->model == 3) f->H_g[l] = H_g_of(p, T);
     f->H_r[l] = H_r_of(p, T);
 }
 
@@ -230,8 +257,13 @@ static void p_xi_2(const oracle_params* p, oracle_fields* f, size_t l) {
         P_k = p->P_d * pow((1. - f->S_n[l] - p->S_wr) / (1. - p->S_wr), -1. / lam);
     }
     f->P_n[l] = f->P_w[l] + P_k;
-    f->Xi_w[l] = -1. * (f->K[l]) * k_w / p->mu_w;
-    f->Xi_n[l] = -1. * (f->K[l]) * k_n / p->mu_n;
+    double mu_w = p->mu_w, mu_n = p->mu_n;
+    if (p->energy) { /* the configuration's mu(T) */
+        mu_w = mu_of_T(p, p->mu_w, f->T[l]);
+        mu_n = mu_of_T(p, p->mu_n, f->T[l]);
+    }
+    f->Xi_w[l] = -1. * (f->K[l]) * k_w / mu_w;
+    f->Xi_n[l] = -1. * (f->K[l]) * k_n / mu_n;
 }
 
 static void p_xi_3(const oracle_params* p, oracle_fields* f, int i, int j, int k, size_t l) {
@@ -250,7 +282,12 @@ static void p_xi_3(const oracle_params* p, oracle_fields* f, int i, int j, int k
         k_n = vg_k_n(p->lambda, S_w_e, S_n_e);
     }
     double mu_w = p->mu_w, mu_n = p->mu_n, mu_g = p->mu_g;
-    if (p->energy) { /* three-phase.cpp:212-214 */
+    if (p->energy && cfg(p)) {
+        double T = f->T[l];
+        mu_w = mu_of_T(p, p->mu_w, T);
+        mu_n = mu_of_T(p, p->mu_n, T);
+        mu_g = mu_of_T(p, p->mu_g, T);
+    } else if (p->energy) { /* three-phase.cpp:212-214 */
         double T = f->T[l];
         mu_w = 1. / (29.21 * T - 7506.64);
         mu_n = 7.256E-10 * exp(4141.9 / T);
@@ -281,6 +318,10 @@ static void assign_ro(const oracle_params* p, oracle_fields* f, size_t l) {
             f->ro_n[l] = p->ro0_n * (1. + (p->beta_n) * (f->P_n[l] - p->P_atm));
             f->ro_g[l] = p->ro0_g * f->P_g[l] / p->P_atm;
         }
+    } else if (p->energy) { /* energy.cpp:146-156's densities, two phases */
+        double T0 = 273;
+        f->ro_w[l] = p->ro0_w * (1. + (p->beta_w) * (f->P_w[l] - p->P_atm) - alfa_w * (f->T[l] - T0));
+        f->ro_n[l] = p->ro0_n * (1. + (p->beta_n) * (f->P_n[l] - p->P_atm) - alfa_n * (f->T[l] - T0));
     } else {
         f->ro_w[l] = p->ro0_w * (1. + (p->beta_w) * (f->P_w[l] - p->P_atm));
         f->ro_n[l] = p->ro0_n * (1. + (p->beta_n) * (f->P_n[l] - p->P_atm));
@@ -423,10 +464,15 @@ void oracle_H_E_current_calculation(const oracle_params* p, oracle_fields* f) {
     OMP_CELLS
     for (size_t l = 0; l < n; ++l) {
         assign_H(p, f, l);
-        f->E[l] = (f->m[l] * (f->S_w[l] * (f->ro_w[l] * f->H_w[l] - f->P_w[l]) +
-                              f->S_n[l] * (f->ro_n[l] * f->H_n[l] - f->P_n[l]) +
-                              f->S_g[l] * (f->ro_g[l] * f->H_g[l] - f->P_g[l])) +
-                   (1. - f->m[l]) * (ro_r * f->H_r[l] - f->P_w[l]));
+        if (p->model == 2) /* assign_E_current without the gas term */
+            f->E[l] = (f->m[l] * (f->S_w[l] * (f->ro_w[l] * f->H_w[l] - f->P_w[l]) +
+                                  f->S_n[l] * (f->ro_n[l] * f->H_n[l] - f->P_n[l])) +
+                       (1. - f->m[l]) * (rock_ro(p) * f->H_r[l] - f->P_w[l]));
+        else
+            f->E[l] = (f->m[l] * (f->S_w[l] * (f->ro_w[l] * f->H_w[l] - f->P_w[l]) +
+                                  f->S_n[l] * (f->ro_n[l] * f->H_n[l] - f->P_n[l]) +
+                                  f->S_g[l] * (f->ro_g[l] * f->H_g[l] - f->P_g[l])) +
+                       (1. - f->m[l]) * (rock_ro(p) * f->H_r[l] - f->P_w[l]));
     }
 }
 
@@ -434,18 +480,27 @@ static double T_flow_at(const oracle_params* p, const oracle_fields* f, size_t l
     double T_flow = 0;
     size_t sx = 1, sy = (size_t)p->Nx, sz = (size_t)p->Nx * p->Ny;
     if (p->Nx > 2)
-        T_flow += (lambda_eff(f, l + sx) * f->T[l + sx] - 2 * lambda_eff(f, l) * f->T[l] +
-                   lambda_eff(f, l - sx) * f->T[l - sx]) / ((p->hx) * (p->hx));
+        T_flow += (lambda_eff(p, f, l + sx) * f->T[l + sx] - 2 * lambda_eff(p, f, l) * f->T[l] +
+                   lambda_eff(p, f, l - sx) * f->T[l - sx]) / ((p->hx) * (p->hx));
     if (p->Ny > 2)
-        T_flow += (lambda_eff(f, l + sy) * f->T[l + sy] - 2 * lambda_eff(f, l) * f->T[l] +
-                   lambda_eff(f, l - sy) * f->T[l - sy]) / ((p->hy) * (p->hy));
-    if (p->Nz > 2) /* '=' rather than '+=', as energy.cpp:257 has it */
-        T_flow = (lambda_eff(f, l + sz) * f->T[l + sz] - 2 * lambda_eff(f, l) * f->T[l] +
-                  lambda_eff(f, l - sz) * f->T[l - sz]) / ((p->hz) * (p->hz));
+        T_flow += (lambda_eff(p, f, l + sy) * f->T[l + sy] - 2 * lambda_eff(p, f, l) * f->T[l] +
+                   lambda_eff(p, f, l - sy) * f->T[l - sy]) / ((p->hy) * (p->hy));
+    if (p->Nz > 2) {
+        double tz = (lambda_eff(p, f, l + sz) * f->T[l + sz] - 2 * lambda_eff(p, f, l) * f->T[l] +
+                     lambda_eff(p, f, l - sz) * f->T[l - sz]) / ((p->hz) * (p->hz));
+        if (cfg(p))
+            T_flow += tz;
+        else /* '=' rather than '+=', as energy.cpp:257 has it */
+            T_flow = tz;
+    }
     return T_flow;
 }
 
-static double adv(const oracle_fields* f, const double* uw, const double* un, const double* ug, size_t a, size_t b) {
+static double adv(const oracle_params* p, const oracle_fields* f, const double* uw, const double* un,
+                  const double* ug, size_t a, size_t b) {
+    if (p->model == 2) /* energy.cpp:279-306's products without the gas terms */
+        return f->ro_w[a] * f->H_w[a] * uw[a] - f->ro_w[b] * f->H_w[b] * uw[b] + f->ro_n[a] * f->H_n[a] * un[a] -
+               f->ro_n[b] * f->H_n[b] * un[b];
     return f->ro_w[a] * f->H_w[a] * uw[a] - f->ro_w[b] * f->H_w[b] * uw[b] + f->ro_n[a] * f->H_n[a] * un[a] -
            f->ro_n[b] * f->H_n[b] * un[b] + f->ro_g[a] * f->H_g[a] * ug[a] - f->ro_g[b] * f->H_g[b] * ug[b];
 }
@@ -453,10 +508,14 @@ static double adv(const oracle_fields* f, const double* uw, const double* un, co
 static double E_flow_at(const oracle_params* p, const oracle_fields* f, size_t l) {
     double E_flow = 0;
     size_t sx = 1, sy = (size_t)p->Nx, sz = (size_t)p->Nx * p->Ny;
-    if (p->Nx > 2) E_flow += adv(f, f->ux_w, f->ux_n, f->ux_g, l + sx, l - sx) / (2. * (p->hx));
-    if (p->Ny > 2) E_flow += adv(f, f->uy_w, f->uy_n, f->uy_g, l + sy, l - sy) / (2. * (p->hy));
-    /* the z term advects with uy, as energy.cpp:300-305 has it */
-    if (p->Nz > 2) E_flow += adv(f, f->uy_w, f->uy_n, f->uy_g, l + sz, l - sz) / (2. * (p->hz));
+    if (p->Nx > 2) E_flow += adv(p, f, f->ux_w, f->ux_n, f->ux_g, l + sx, l - sx) / (2. * (p->hx));
+    if (p->Ny > 2) E_flow += adv(p, f, f->uy_w, f->uy_n, f->uy_g, l + sy, l - sy) / (2. * (p->hy));
+    if (p->Nz > 2) {
+        if (cfg(p))
+            E_flow += adv(p, f, f->uz_w, f->uz_n, f->uz_g, l + sz, l - sz) / (2. * (p->hz));
+        else /* the z term advects with uy, as energy.cpp:300-305 has it */
+            E_flow += adv(p, f, f->uy_w, f->uy_n, f->uy_g, l + sz, l - sz) / (2. * (p->hz));
+    }
     return E_flow;
 }
 
@@ -586,7 +645,7 @@ static void newton_energy(const oracle_params* p, oracle_fields* f, size_t l) {
         F[2] = rn * Sn - f->roS_n[l];
         F[3] = rg * Sg - f->roS_g[l];
         F[4] = m * (Sw * (rw * f->H_w[l] - P) + Sn * (rn * f->H_n[l] - P) + Sg * (rg * f->H_g[l] - P)) +
-               (1. - m) * (ro_r * f->H_r[l] - P) - f->E_new[l];
+               (1. - m) * (rock_ro(p) * f->H_r[l] - P) - f->E_new[l];
         dF[0] = 1.; dF[1] = 1.; dF[2] = 1.; dF[3] = 0.; dF[4] = 0.;
         dF[0 + n] = rw; dF[1 + n] = 0.; dF[2 + n] = 0.;
         dF[3 + n] = d_ro_e(p, P, T, 'w', 'P') * Sw;
@@ -607,7 +666,7 @@ static void newton_energy(const oracle_params* p, oracle_fields* f, size_t l) {
         dF[4 + n * 4] = m * (Sw * (d_ro_e(p, P, T, 'w', 'T') * f->H_w[l] + rw * c_w(T)) +
                              Sn * (d_ro_e(p, P, T, 'n', 'T') * f->H_n[l] + rn * c_n(T)) +
                              Sg * (d_ro_e(p, P, T, 'g', 'T') * f->H_g[l] + rg * c_g(T))) +
-                        (1. - m) * ro_r * c_r(T);
+                        (1. - m) * rock_ro(p) * rock_c(p, T);
         oracle_reverse_matrix(dF, n);
         oracle_mult_matrix_vector(corr, dF, F, n);
         f->S_w[l] = Sw - corr[0];
@@ -620,6 +679,48 @@ static void newton_energy(const oracle_params* p, oracle_fields* f, size_t l) {
     f->E[l] = f->E_new[l];
 }
 
+/* The two-phase energy model's Newton (configuration closure; no reference
+ * counterpart): energy.cpp:370-463's system without the gas row and column,
+ * a 4x4 system in (S_w, S_n, P_w, T) -- saturations summing to one, the two
+ * mass balances and the energy balance, densities at P_w for both phases as
+ * energy.cpp:388-394 has them -- inverted by reverse_matrix(dF, 4). */
+static void newton_energy2(const oracle_params* p, oracle_fields* f, size_t l) {
+    const int n = 4;
+    double F[4], corr[4], dF[16];
+    for (int w = 1; w <= p->iterations; w++) {
+        double P = f->P_w[l], T = f->T[l], Sw = f->S_w[l], Sn = f->S_n[l], m = f->m[l];
+        double rw = ro_e(p, P, T, 'w'), rn = ro_e(p, P, T, 'n');
+        F[0] = Sw + Sn - 1.;
+        F[1] = rw * Sw - f->roS_w[l];
+        F[2] = rn * Sn - f->roS_n[l];
+        F[3] = m * (Sw * (rw * f->H_w[l] - P) + Sn * (rn * f->H_n[l] - P)) + (1. - m) * (rock_ro(p) * f->H_r[l] - P) -
+               f->E_new[l];
+        dF[0] = 1.; dF[1] = 1.; dF[2] = 0.; dF[3] = 0.;
+        dF[0 + n] = rw; dF[1 + n] = 0.;
+        dF[2 + n] = d_ro_e(p, P, T, 'w', 'P') * Sw;
+        dF[3 + n] = d_ro_e(p, P, T, 'w', 'T') * Sw;
+        dF[0 + n * 2] = 0.; dF[1 + n * 2] = rn;
+        dF[2 + n * 2] = d_ro_e(p, P, T, 'n', 'P') * Sn;
+        dF[3 + n * 2] = d_ro_e(p, P, T, 'n', 'T') * Sn;
+        dF[0 + n * 3] = m * (rw * f->H_w[l] - P);
+        dF[1 + n * 3] = m * (rn * f->H_n[l] - P);
+        dF[2 + n * 3] = m * (Sw * (d_ro_e(p, P, T, 'w', 'P') * f->H_w[l] - 1.) +
+                             Sn * (d_ro_e(p, P, T, 'n', 'P') * f->H_n[l] - 1.)) +
+                        (1. - m) * (-1);
+        dF[3 + n * 3] = m * (Sw * (d_ro_e(p, P, T, 'w', 'T') * f->H_w[l] + rw * c_w(T)) +
+                             Sn * (d_ro_e(p, P, T, 'n', 'T') * f->H_n[l] + rn * c_n(T))) +
+                        (1. - m) * rock_ro(p) * rock_c(p, T);
+        oracle_reverse_matrix(dF, n);
+        oracle_mult_matrix_vector(corr, dF, F, n);
+        f->S_w[l] = Sw - corr[0];
+        f->S_n[l] = Sn - corr[1];
+        f->P_w[l] = P - corr[2];
+        f->T[l] = T - corr[3];
+        assign_H(p, f, l);
+    }
+    f->E[l] = f->E_new[l];
+}
+
 void oracle_P_S_calculation(const oracle_params* p, oracle_fields* f) {
     OMP_CELLS
     for (int k = 0; k < p->Nz; ++k)
@@ -627,7 +728,12 @@ void oracle_P_S_calculation(const oracle_params* p, oracle_fields* f) {
             for (int i = 0; i < p->Nx; ++i) {
                 size_t l = IDX(p, i, j, k);
                 if (p->model == 2) {
-                    if (interior_newton2(p, i, j, k)) newton2(p, f, l);
+                    if (interior_newton2(p, i, j, k)) {
+                        if (p->energy)
+                            newton_energy2(p, f, l);
+                        else
+                            newton2(p, f, l);
+                    }
                 } else if (interior(p, i, j, k)) {
                     if (p->energy)
                         newton_energy(p, f, l);
@@ -638,6 +744,18 @@ void oracle_P_S_calculation(const oracle_params* p, oracle_fields* f) {
 }
 
 /* ---- boundary_conditions (cpu.cpp:91-104) ------------------------------------------ */
+
+/* Border_T (energy.cpp:343-366) with the configuration's face temperatures:
+ * top and bottom fixed where given (> 0), else, like the side faces,
+ * insulated. */
+static void border_T_cfg(const oracle_params* p, oracle_fields* f, int j, size_t l, size_t l1) {
+    if (j == 0 && p->T_top > 0)
+        f->T[l] = p->T_top;
+    else if (j == p->Ny - 1 && p->T_bottom > 0)
+        f->T[l] = p->T_bottom;
+    else
+        f->T[l] = f->T[l1];
+}
 
 static void border2(const oracle_params* p, oracle_fields* f, int i, int j, int k) {
     /* two-phase.cpp:163-215 and ro_eff_gdy (main.cpp:255-267) */
@@ -655,10 +773,13 @@ static void border2(const oracle_params* p, oracle_fields* f, int i, int j, int 
         f->P_w[l] = f->P_w[l1];
     else if (j == 0)
         f->P_w[l] = p->P_atm;
+    else if (p->P_bottom > 0) /* the configurations' fixed bottom pressure */
+        f->P_w[l] = p->P_bottom;
     else {
         double ro_g_dy = (f->ro_n[l1] * f->S_n[l1] + f->ro_w[l1] * (1 - f->S_n[l1])) * (f->m[l1]) * (p->g_const) * (p->hy);
         f->P_w[l] = f->P_w[l1] + ro_g_dy;
     }
+    if (p->energy) border_T_cfg(p, f, j, l, l1);
 }
 
 static void border3(const oracle_params* p, oracle_fields* f, int i, int j, int k) {
@@ -687,7 +808,9 @@ static void border3(const oracle_params* p, oracle_fields* f, int i, int j, int 
         f->P_n[l] = (P1 + Pk_nw + (p->ro0_n) * g * hy * (1. - (p->beta_n) * (p->P_atm))) / (1. - (p->beta_n) * (p->ro0_n) * g * hy);
         f->P_g[l] = (P1 + Pk_nw + Pk_gn) / (1. - (p->ro0_g) * g * hy / (p->P_atm));
     }
-    if (p->energy) {
+    if (p->energy && cfg(p)) {
+        border_T_cfg(p, f, j, l, l1);
+    } else if (p->energy) {
         if (j == 0)
             f->T[l] = 400;
         else if (j == p->Ny - 1)

@@ -17,7 +17,8 @@ enum { ORACLE_CLOSURE_REFERENCE = 0, ORACLE_CLOSURE_COREY = 1 };
 
 typedef struct oracle_params {
     int model;      /* 2: water/NAPL, 3: water/NAPL/gas */
-    int energy;     /* 1: energy equation (model 3 only, as in energy.cpp) */
+    int energy;     /* 1: energy equation (energy.cpp for model 3; model 2
+                       with energy_closure 1 only) */
     int closure;    /* ORACLE_CLOSURE_* */
     int Nx, Ny, Nz;
     int source;     /* SOURCE */
@@ -32,6 +33,17 @@ typedef struct oracle_params {
     double P_d;     /* P_D_0 (two-phase) */
     double S_w_gr, S_n_gr;
     double corey_n; /* Corey exponent of the "corey" closure */
+    /* Boundary and energy closures of the benchmark configurations (no
+     * reference counterpart; zero values select the reference's rules). */
+    double P_bottom;    /* two phases: > 0 a fixed bottom-face pressure instead of
+                           the hydrostatic rule of two-phase.cpp:100-107 */
+    int energy_closure; /* 0: energy.cpp's constants, closures and its z-term
+                           quirks (three phases only); 1: the configuration's
+                           closure (below), for two or three phases */
+    double lambda_eff;  /* 1: thermal conductivity of the medium, W/(m K) */
+    double c_r, ro_r;   /* 1: rock heat capacity J/(kg K), rock density kg/m^3 */
+    double T_top, T_bottom; /* 1: face temperatures; <= 0 insulated (copy inward) */
+    double mu_T_a, mu_T_ref; /* 1: mu_a(T) = mu_a * exp(-mu_T_a * (T - mu_T_ref)) */
 } oracle_params;
 
 typedef struct oracle_fields {

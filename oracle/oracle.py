@@ -2,10 +2,12 @@
 
 ``oracle/liboracle.so`` is built from ``capaz_oracle.c`` by ``oracle/Makefile``
 (and by ``__graft_entry__.build()``).  Parameters are given as defines.ini
-keys (main.cpp:1327-1626) plus four keys of the backend's own: ``MODEL``
-(2 or 3), ``ENERGY`` (0/1), ``CLOSURE`` ("reference" or "corey") and
-``COREY_N``.  Only tests, smoke() and the CPU-baseline leg of bench.py use
-this module, to compare against.
+keys (main.cpp:1327-1626) plus keys of the backend's own: ``MODEL``
+(2 or 3), ``ENERGY`` (0/1), ``CLOSURE`` ("reference" or "corey"),
+``COREY_N``, ``P_BOTTOM`` and the energy closure of the configurations
+(``ENERGY_CLOSURE`` "config" with ``LAMBDA_EFF``, ``C_R``, ``RO_R``,
+``T_TOP``, ``T_BOTTOM``, ``MU_T_A``, ``MU_T_REF``).  Only tests, smoke()
+and the CPU-baseline leg of bench.py use this module, to compare against.
 """
 from __future__ import annotations
 
@@ -42,6 +44,10 @@ class _Params(ctypes.Structure):
         ("P_d", ctypes.c_double),
         ("S_w_gr", ctypes.c_double), ("S_n_gr", ctypes.c_double),
         ("corey_n", ctypes.c_double),
+        ("P_bottom", ctypes.c_double), ("energy_closure", ctypes.c_int),
+        ("lambda_eff", ctypes.c_double), ("c_r", ctypes.c_double), ("ro_r", ctypes.c_double),
+        ("T_top", ctypes.c_double), ("T_bottom", ctypes.c_double),
+        ("mu_T_a", ctypes.c_double), ("mu_T_ref", ctypes.c_double),
     ]
 
 
@@ -98,6 +104,11 @@ def params_from_keys(keys: dict) -> _Params:
     p.P_d = g("P_D_0")
     p.S_w_gr, p.S_n_gr = g("S_W_GR"), g("S_N_GR")
     p.corey_n = g("COREY_N", 2.0)
+    p.P_bottom = g("P_BOTTOM")
+    p.energy_closure = 1 if keys.get("ENERGY_CLOSURE", "reference") == "config" else 0
+    p.lambda_eff, p.c_r, p.ro_r = g("LAMBDA_EFF"), g("C_R"), g("RO_R")
+    p.T_top, p.T_bottom = g("T_TOP"), g("T_BOTTOM")
+    p.mu_T_a, p.mu_T_ref = g("MU_T_A"), g("MU_T_REF")
     return p
 
 
@@ -107,7 +118,7 @@ def model_fields(keys: dict) -> list[str]:
     if model == 3:
         out += list(THREE_PHASE_FIELDS)
     if energy:
-        out += list(ENERGY_FIELDS)
+        out += [f for f in ENERGY_FIELDS if model == 3 or f != "H_g"]
     return out
 
 

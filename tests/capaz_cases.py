@@ -30,7 +30,8 @@ REF_3PH = dict(
 )
 
 # Keys of the backend only (not defines.ini keys of the reference).
-BACKEND_KEYS = ("MODEL", "ENERGY", "CLOSURE", "COREY_N")
+BACKEND_KEYS = ("MODEL", "ENERGY", "CLOSURE", "COREY_N", "P_BOTTOM", "ENERGY_CLOSURE", "LAMBDA_EFF", "C_R", "RO_R",
+                "T_TOP", "T_BOTTOM", "MU_T_A", "MU_T_REF")
 
 
 def ref_keys(keys: dict) -> dict:
@@ -39,7 +40,14 @@ def ref_keys(keys: dict) -> dict:
 
 
 def case(model: str, nx: int, ny: int, nz: int, **over) -> dict:
-    """Reference deck for ``model`` ("2ph", "3ph", "3phe") on an nx*ny*nz grid."""
+    """Reference deck for ``model`` ("2ph", "3ph", "3phe") on an nx*ny*nz grid;
+    "2phe" is the non-isothermal two-phase workload's physics (configuration
+    closures, no reference deck)."""
+    if model == "2phe":
+        from capaz_amd import workloads
+        k, _ = workloads.nonisothermal_2ph(nx, ny, nz)
+        k.update(over)
+        return k
     if model == "2ph":
         k = dict(REF_2PH)
     else:
@@ -75,6 +83,8 @@ def random_state(keys: dict, seed: int, hetero_m: bool = True) -> dict[str, np.n
     st["K"] = np.full(shape, keys["K_0"])
     if model == 2:
         st["S_n"] = rng.uniform(0.05, 0.4, shape)
+        if int(keys.get("ENERGY", 0)):
+            st["T"] = 293.0 + rng.uniform(0, 30, shape)
     else:
         st["S_w"] = rng.uniform(0.25, 0.5, shape)
         st["S_n"] = rng.uniform(0.15, 0.35, shape)

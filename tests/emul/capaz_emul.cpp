@@ -34,7 +34,8 @@ int capaz_emul_entry(const capaz_params* u, double** fields, int entry) {
         for (int64_t l = 0; l < p.n; ++l) {
             int i, j, k;
             capaz::decompose(p, l, i, j, k);
-            redo[l] = capaz::interior3(p, i, j, k) && !capaz::body_newton_energy_nopivot(p, F, l);
+            redo[l] = p.model == 2 ? capaz::interior_newton2(p, i, j, k) && !capaz::body_newton_energy2_nopivot(p, F, l)
+                                   : capaz::interior3(p, i, j, k) && !capaz::body_newton_energy_nopivot(p, F, l);
         }
         for (int64_t l = 0; l < p.n; ++l) {
             if (!redo[l]) continue;
@@ -127,6 +128,8 @@ int capaz_emul_reverse_matrix(double* a, int n) {
     if (n == 3) return capaz::reverse_matrix<3>(a);
     if (n == 5) return capaz::reverse_matrix<5>(a);
     if (n == -5) return capaz::reverse_nopivot<5>(a) ? 0 : 1;  // 1: refused, `a` untouched
+    if (n == 4) return capaz::reverse_matrix<4>(a);
+    if (n == -14) return capaz::reverse_nopivot<4>(a) ? 0 : 1;  // the 4x4 no-pivot path
     return -2;
 }
 

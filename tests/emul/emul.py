@@ -50,11 +50,15 @@ def validate(keys: dict) -> str:
 
 def reverse_matrix(a: np.ndarray, fast: bool = False, a02zero: bool = False) -> tuple[int, np.ndarray]:
     """The kernels' reverse_matrix<n>; fast=True runs the 3x3 no-pivot path
-    with its fallback (for n = 5 the no-pivot routine alone, rc 1 when it
+    with its fallback (for n = 4, 5 the no-pivot routine alone, rc 1 when it
     refuses), a02zero=True its variant for a[0,2] == +0 (the one newton3
     uses)."""
     buf = np.ascontiguousarray(a, dtype=np.float64).copy()
-    n = -4 if a02zero else (-a.shape[0] if fast else a.shape[0])
+    n = a.shape[0]
+    if a02zero:
+        n = -4
+    elif fast:
+        n = -14 if n == 4 else -n
     rc = lib().capaz_emul_reverse_matrix(buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
     return rc, buf
 

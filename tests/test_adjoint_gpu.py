@@ -6,8 +6,9 @@ capaz_adjoint).
     closure), and the gradient matches the host build of the same adjoint
     bodies to rounding;
   * the gradient does not depend on the checkpoint interval (each segment
-    is recomputed with the same bits), and the default interval records the
-    whole trajectory when it fits;
+    is recomputed with the same bits) nor on whether the recompute runs on a
+    second stream, and the default interval records the whole trajectory
+    when it fits;
   * directional derivatives match central differences of capaz_objective;
   * afterwards the context holds the state after nsteps, as capaz_step;
   * the tangent-linear model (capaz_tangent) gives the host build's J and dJ,
@@ -53,15 +54,18 @@ def test_adjoint_matches_host_build_and_checkpointing_is_exact(kernel, dims):
     d.tune(adjoint_kernel=kernel)
     assert d.objective(9, obs) == J_h
     grads = []
-    for every, segment in ((0, 9), (-1, 3), (1, 1), (2, 2), (4, 4), (9, 9)):
-        for f, v in st.items():
-            d.set_field(f, v)
-        J, gK, gm = d.adjoint(9, obs, checkpoint_every=every)
-        # 0: the whole trajectory (a small grid fits), < 0: round(sqrt(9))
-        assert d.tuning()["adjoint_segment"] == segment
-        assert J == J_h
-        assert rel(gK, gK_h) < 1e-10 and rel(gm, gm_h) < 1e-10, f"checkpoint_every={every}"
-        grads.append((gK, gm))
+    # two streams (the segment recompute beside the passes) and one
+    for streams in (2, 1):
+        d.tune(adjoint_streams=streams)
+        for every, segment in ((0, 9), (-1, 3), (1, 1), (2, 2), (4, 4), (9, 9)):
+            for f, v in st.items():
+                d.set_field(f, v)
+            J, gK, gm = d.adjoint(9, obs, checkpoint_every=every)
+            # 0: the whole trajectory (a small grid fits), < 0: round(sqrt(9))
+            assert d.tuning()["adjoint_segment"] == segment
+            assert J == J_h
+            assert rel(gK, gK_h) < 1e-10 and rel(gm, gm_h) < 1e-10, f"checkpoint_every={every} streams={streams}"
+            grads.append((gK, gm))
     for gK, gm in grads[1:]:
         assert bit_equal(gK, grads[0][0]) and bit_equal(gm, grads[0][1])
     # the context holds the state after 9 steps, intermediates included

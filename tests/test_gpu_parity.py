@@ -371,6 +371,61 @@ def test_face_buffers_match_reference_layout():
     buf.close()
 
 
+@pytest.mark.parametrize("dims,offset", [((9, 8, 7), 0), ((9, 7, 6), 0), ((9, 8, 7), 8)])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_batched_faces_equal_the_per_face_calls(axis, dims, offset):
+    """capaz_load_faces packs every field's two exchange planes in one launch,
+    [field][side][plane], as the per-field, per-side capaz_load_face calls
+    lay them out; capaz_save_faces writes the first and last planes as
+    capaz_save_face does.  After fused steps (ping-pong state, stale
+    intermediates) both see the current state.  z planes of an even size in a
+    16-byte aligned buffer take the two-doubles-per-lane path; an odd plane
+    (9 x 7) and a buffer 8 bytes off alignment take the one-double path."""
+    import ctypes
+    from capaz_amd import hip
+    nx, ny, nz = dims
+    keys = case("3ph", nx, ny, nz, CLOSURE="corey", S_WR_0=0.1, S_NR_0=0.05)
+    st = random_state(keys, 4)
+    d = capaz.Domain(keys, st)
+    d.run(3)
+    L = capaz.load()
+    names = ("P_w", "S_w", "S_n", "P_n", "P_g")
+    ids = (ctypes.c_int * 5)(*[capaz.FIELD_ID[f] for f in names])
+    plane = {0: ny * nz, 1: nx * nz, 2: nx * ny}[axis]
+    big, one = hip.DeviceBuffer(8 * plane * 10 + offset), hip.DeviceBuffer(8 * plane)
+    at = ctypes.c_void_p(big.address + offset)
+
+    def download_big():
+        raw = big.download(plane * 10 + offset // 8)
+        return raw[offset // 8:].reshape(5, 2, plane)
+
+    assert L.capaz_load_faces(d._ctx, ids, 5, axis, at) == 0
+    d.sync()
+    got = download_big()
+    for q, f in enumerate(names):
+        for s in (0, 1):
+            assert L.capaz_load_face(d._ctx, ids[q], axis, s, ctypes.c_void_p(one.address)) == 0
+            d.sync()
+            np.testing.assert_array_equal(got[q, s], one.download(plane))
+    # unpack a known buffer and compare with the per-face writes on a twin
+    e = capaz.Domain(keys, st)
+    e.run(3)
+    vals = np.arange(plane * 10, dtype=np.float64).reshape(5, 2, plane) + 0.25
+    big.upload(np.concatenate([np.zeros(offset // 8), vals.ravel()]))
+    assert L.capaz_save_faces(d._ctx, ids, 5, axis, at) == 0
+    for q in range(5):
+        for s in (0, 1):
+            one.upload(vals[q, s])
+            assert L.capaz_save_face(e._ctx, ids[q], axis, s, ctypes.c_void_p(one.address)) == 0
+            e.sync()
+    for f in names:
+        assert bit_equal(d.field(f), e.field(f)), f
+    big.close()
+    one.close()
+    d.close()
+    e.close()
+
+
 def test_invalid_field_reports_error():
     d = capaz.Domain(case("2ph", 5, 5, 5), None)
     with pytest.raises(capaz.CapazError, match="does not exist"):
