@@ -68,7 +68,28 @@ CAPAZ_HD double rcp_rn(double b) {
 #endif
 }
 
+// rcp_rn for a pivot the caller has checked to satisfy |b| >= 1e-8: only the
+// upper end of the range (and NaN) can take the IEEE division.
+CAPAZ_HD double rcp_rn_pivot(double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double ab = fabs(b);
+    double y = __builtin_amdgcn_rcp(b);
+    double e = fma(-b, y, 1.);
+    y = fma(e, y, y);
+    e = fma(-b, y, 1.);
+    y = fma(e, y, y);
+    e = fma(-b, y, 1.);
+    y = fma(e, y, y);
+    if (__builtin_expect(!(ab <= 0x1p1000), 0)) y = rcp_ieee(b);
+    return y;
+#else
+    return 1. / b;
+#endif
+}
+
 CAPAZ_HD double half(double x) { return x * 0.5; }
+// x clipped to [0, 1]; a NaN passes through (so a non-finite state spreads
+// through k_r as in the oracle; fmin/fmax would turn it into 0).
 CAPAZ_HD double clamp01(double x) { return x < 0. ? 0. : (x > 1. ? 1. : x); }
 
 // ---- grid predicates ------------------------------------------------------------
@@ -312,7 +333,7 @@ CAPAZ_HD bool reverse3_nopivot(double* a) {
     // k = 0: row 0 = [1, a01/f, a02/f | 1/f, 0, 0]
     double f = a[0];
     if (fabs(f) < e) return false;
-    double rf = rcp_rn(f);
+    double rf = rcp_rn_pivot(f);
     double r01 = exact_div(a[1], f, rf), r02 = exact_div(a[2], f, rf), r03 = rf;
     // rows 1, 2 lose column 0: [0, b_i1, b_i2 | b_i3, delta_i1, delta_i2]
     double f1 = a[3];
@@ -322,7 +343,7 @@ CAPAZ_HD bool reverse3_nopivot(double* a) {
     // k = 1: row 1 = [0, 1, b12/f | b13/f, 1/f, 0]
     f = b11;
     if (fabs(f) < e) return false;
-    rf = rcp_rn(f);
+    rf = rcp_rn_pivot(f);
     double s12 = exact_div(b12, f, rf), s13 = exact_div(b13, f, rf), s14 = rf;
     f1 = r01;  // row 0
     double c02 = r02 - s12 * f1, c03 = r03 - s13 * f1, c04 = 0. - s14 * f1;
@@ -331,7 +352,7 @@ CAPAZ_HD bool reverse3_nopivot(double* a) {
     // k = 2: row 2 = [0, 0, 1 | c23/f, c24/f, 1/f]
     f = c22;
     if (fabs(f) < e) return false;
-    rf = rcp_rn(f);
+    rf = rcp_rn_pivot(f);
     double t23 = exact_div(c23, f, rf), t24 = exact_div(c24, f, rf), t25 = rf;
     f1 = c02;  // row 0
     double o03 = c03 - t23 * f1, o04 = c04 - t24 * f1, o05 = 0. - t25 * f1;
@@ -356,33 +377,36 @@ inline int reverse_matrix3_general(double* a) { return reverse_matrix<3>(a); }
 
 // reverse3_nopivot for a matrix whose a[2] is +0 (the three-phase Jacobian's
 // dF[2], three-phase.cpp:321): a02/f is +-0, so b12 = a12 - (+-0)*f1 = a12,
-// b22 = a22 and c02 = (+-0) - s12*r01 = 0. - s12*r01, all exact.
+// b22 = a22 and c02 = (+-0) - s12*r01 = -(s12*r01), all exact.  The
+// eliminations against a zero of the identity block, 0. - x, are written
+// -(x): the same value up to the sign of an exact zero, which no non-zero
+// entry of the inverse or of the Newton update depends on.
 CAPAZ_HD bool reverse3_nopivot_a02zero(double* a) {
     const double e = 1E-8;
     double f = a[0];
     if (fabs(f) < e) return false;
-    double rf = rcp_rn(f);
+    double rf = rcp_rn_pivot(f);
     double r01 = exact_div(a[1], f, rf), r03 = rf;
     double f1 = a[3];
-    double b11 = a[4] - r01 * f1, b12 = a[5], b13 = 0. - r03 * f1;
+    double b11 = a[4] - r01 * f1, b12 = a[5], b13 = -(r03 * f1);
     double f2 = a[6];
-    double b21 = a[7] - r01 * f2, b22 = a[8], b23 = 0. - r03 * f2;
+    double b21 = a[7] - r01 * f2, b22 = a[8], b23 = -(r03 * f2);
     f = b11;
     if (fabs(f) < e) return false;
-    rf = rcp_rn(f);
+    rf = rcp_rn_pivot(f);
     double s12 = exact_div(b12, f, rf), s13 = exact_div(b13, f, rf), s14 = rf;
     f1 = r01;
-    double c02 = 0. - s12 * f1, c03 = r03 - s13 * f1, c04 = 0. - s14 * f1;
+    double c02 = -(s12 * f1), c03 = r03 - s13 * f1, c04 = -(s14 * f1);
     f2 = b21;
-    double c22 = b22 - s12 * f2, c23 = b23 - s13 * f2, c24 = 0. - s14 * f2;
+    double c22 = b22 - s12 * f2, c23 = b23 - s13 * f2, c24 = -(s14 * f2);
     f = c22;
     if (fabs(f) < e) return false;
-    rf = rcp_rn(f);
+    rf = rcp_rn_pivot(f);
     double t23 = exact_div(c23, f, rf), t24 = exact_div(c24, f, rf), t25 = rf;
     f1 = c02;
-    double o03 = c03 - t23 * f1, o04 = c04 - t24 * f1, o05 = 0. - t25 * f1;
+    double o03 = c03 - t23 * f1, o04 = c04 - t24 * f1, o05 = -(t25 * f1);
     f2 = s12;
-    double o13 = s13 - t23 * f2, o14 = s14 - t24 * f2, o15 = 0. - t25 * f2;
+    double o13 = s13 - t23 * f2, o14 = s14 - t24 * f2, o15 = -(t25 * f2);
     a[0] = o03; a[1] = o04; a[2] = o05;
     a[3] = o13; a[4] = o14; a[5] = o15;
     a[6] = t23; a[7] = t24; a[8] = t25;
@@ -521,7 +545,9 @@ CAPAZ_HD void mult_matrix_vector(double* r, const double* m, const double* v) {
 // At step k those are the left columns after k and the right (inverse)
 // columns up to N + k: the left columns up to k are never read again, and a
 // right column after N + k is still the identity's (zero outside row k's
-// divided zero), so the skipped updates only subtract zeros.  The result has
+// divided zero), so the skipped updates only subtract zeros.  Column N + k
+// is still the identity's at step k: its row-k entry 1 / f is RN(1/f) = rf
+// itself, and its other entries 0 - rf * f1 are -(rf * f1).  The result has
 // the general routine's bits up to the sign of entries that are exactly 0.
 // Returns false, leaving `a` untouched, if some pivot has |f| < 1e-8.
 template <int N>
@@ -540,15 +566,17 @@ CAPAZ_HD bool reverse_nopivot(double* a) {
     for (int k = 0; k < N; k++) {
         const double f = m[k][k];
         if (fabs(f) < e) return false;
-        const double rf = rcp_rn(f);  // RN(1/f), as the general routine's 1. / f
+        const double rf = rcp_rn_pivot(f);  // RN(1/f), as the general routine's 1. / f
 #pragma unroll
-        for (int j = k + 1; j <= N + k; j++) m[k][j] = exact_div(m[k][j], f, rf);
+        for (int j = k + 1; j < N + k; j++) m[k][j] = exact_div(m[k][j], f, rf);
+        m[k][N + k] = rf;
 #pragma unroll
         for (int i = 0; i < N; i++) {
             if (i != k) {
                 const double f1 = m[i][k];
 #pragma unroll
-                for (int j = k + 1; j <= N + k; j++) m[i][j] -= (m[k][j] * f1);
+                for (int j = k + 1; j < N + k; j++) m[i][j] -= (m[k][j] * f1);
+                m[i][N + k] = -(rf * f1);
             }
         }
     }
