@@ -1,4 +1,5 @@
-// Discrete adjoint of the fused three-phase step (Corey closures), per cell.
+// Discrete adjoint of the fused three-phase step (Corey closures and the
+// reference's van Genuchten closures), per cell.
 //
 // The forward step maps the state x_t (P_w, S_w, S_n in every cell; P_n, P_g
 // in the boundary cells, where they persist) and the parameters (K, m) to
@@ -47,7 +48,9 @@ struct AdjIO {
     double *g_lnK, *g_m;                                  // accumulated gradients
 };
 
-// Corey k_r and dk_r/ds at an effective saturation.
+// Corey k_r and dk_r/ds at an effective saturation; N2: the exponent is 2
+// (no pow in the code).
+template <bool N2 = false>
 CAPAZ_HD void corey_d(const DevParams& p, double s, double& kr, double& dkr) {
     if (s <= 0.) {
         kr = 0.;
@@ -55,13 +58,74 @@ CAPAZ_HD void corey_d(const DevParams& p, double s, double& kr, double& dkr) {
     } else if (s >= 1.) {
         kr = 1.;
         dkr = 0.;
-    } else if (p.corey_n == 2.) {
+    } else if (N2 || p.corey_n == 2.) {
         kr = s * s;
         dkr = 2. * s;
     } else {
         kr = pow(s, p.corey_n);
         dkr = p.corey_n * pow(s, p.corey_n - 1.);
     }
+}
+
+// The reference's van Genuchten k_r (three-phase.cpp:146-186, as vg_k_w,
+// vg_k_n and vg_k_g evaluate them) and their partial derivatives in the
+// effective saturations x = Swe, y = Sne (z = Sge = 1 - x - y), by the chain
+// rule on each power (a = p.vg_a, b = p.vg_b, c = p.vg_c):
+//   k_w = x^(1/2) t^2,           t = 1 - (1 - x^a)^b        (x >= 1e-3)
+//   k_g = z^(1/2) v^c,           v = 1 - (1 - z)^a          (z >= 1e-3)
+//   k_n = y k_nw k_ng / D,       D = (1 - x)(x + y)         (y >= 1e-3)
+//         k_nw = (1 - x)^(1/2) (1 - x^a)^c,  k_ng = y^(1/2) s^2,  s = 1 - (1 - y^a)^b.
+CAPAZ_HD void vg_d(const DevParams& p, double x, double y, double& kw, double& kn, double& kg, double& dkw_dx,
+                   double& dkn_dx, double& dkn_dy, double& dkg_dz) {
+    const double a = p.vg_a, b = p.vg_b, c = p.vg_c;
+    const double z = 1. - x - y;
+    kw = kn = kg = dkw_dx = dkn_dx = dkn_dy = dkg_dz = 0.;
+    if (x >= 1e-3) {
+        const double u = 1. - pow(x, a), t = 1. - pow(u, b);
+        const double dt = b * pow(u, b - 1.) * a * pow(x, a - 1.);
+        const double sx = sqrt(x);
+        kw = sx * t * t;
+        dkw_dx = 0.5 / sx * t * t + 2. * sx * t * dt;
+    }
+    if (z >= 1e-3) {
+        const double v = 1. - pow(1. - z, a), vc = pow(v, c);
+        const double dv = a * pow(1. - z, a - 1.);
+        const double sz = sqrt(z);
+        kg = sz * vc;
+        dkg_dz = 0.5 / sz * vc + sz * c * pow(v, c - 1.) * dv;
+    }
+    if (y >= 1e-3) {
+        const double w = 1. - pow(x, a), sw = sqrt(1. - x);
+        const double knw = sw * pow(w, c);
+        const double dknw = -0.5 / sw * pow(w, c) - sw * c * pow(w, c - 1.) * a * pow(x, a - 1.);
+        const double q = 1. - pow(y, a), s = 1. - pow(q, b);
+        const double ds = b * pow(q, b - 1.) * a * pow(y, a - 1.);
+        const double sy = sqrt(y);
+        const double kng = sy * s * s;
+        const double dkng = 0.5 / sy * s * s + 2. * sy * s * ds;
+        const double D = (1. - x) * (x + y);
+        kn = y * knw * kng / D;
+        dkn_dx = y * kng * (dknw * D - knw * (1. - 2. * x - y)) / (D * D);
+        dkn_dy = knw * ((kng + y * dkng) * D - y * kng * (1. - x)) / (D * D);
+    }
+}
+
+// k_r of the model's closure (Corey or the reference's van Genuchten) and
+// their partial derivatives: dkw = dk_w/dSwe, dkn_w / dkn_n = dk_n/dSwe /
+// dSne, dkg = dk_g/dSge.  CL: 1 Corey, 0 van Genuchten, -1 p.closure at run
+// time, 2 Corey with the exponent 2 (a kernel that knows its closure leaves
+// the others' code out: the van Genuchten registers and the device pow).
+template <int CL = -1>
+CAPAZ_HD void kr3_d(const DevParams& p, double Swe, double Sne, double& kw, double& kn, double& kg, double& dkw,
+                    double& dkn_w, double& dkn_n, double& dkg) {
+    if (CL == 0 || (CL < 0 && p.closure == 0)) {
+        vg_d(p, Swe, Sne, kw, kn, kg, dkw, dkn_w, dkn_n, dkg);
+        return;
+    }
+    corey_d<CL == 2>(p, Swe, kw, dkw);
+    corey_d<CL == 2>(p, Sne, kn, dkn_n);
+    corey_d<CL == 2>(p, 1. - Swe - Sne, kg, dkg);
+    dkn_w = 0.;
 }
 
 // A cell's coefficients and their derivatives with respect to its own state.
@@ -72,7 +136,7 @@ struct CoefD {
     bool inner;  // P_n = P_g = P_w (three-phase.cpp:225-235)
 };
 
-template <class IO>
+template <int CL = -1, class IO>
 CAPAZ_HD CoefD coef_d(const DevParams& p, const IO& a, int64_t l, bool inner) {
     CoefD c;
     const double Pw = a.Pw[l], Sw = a.Sw[l], Sn = a.Sn[l], K = a.K[l];
@@ -81,18 +145,15 @@ CAPAZ_HD CoefD coef_d(const DevParams& p, const IO& a, int64_t l, bool inner) {
     c.P[1] = inner ? Pw : a.Pn[l];
     c.P[2] = inner ? Pw : a.Pg[l];
     const double Swe = (Sw - p.S_wr) * p.r_den3, Sne = (Sn - p.S_nr) * p.r_den3;
-    const double Sge = 1. - Swe - Sne;
-    double kw, dkw, kn, dkn, kg, dkg;
-    corey_d(p, Swe, kw, dkw);
-    corey_d(p, Sne, kn, dkn);
-    corey_d(p, Sge, kg, dkg);
+    double kw, dkw, kn, dkn_w, dkn_n, kg, dkg;
+    kr3_d<CL>(p, Swe, Sne, kw, kn, kg, dkw, dkn_w, dkn_n, dkg);
     c.Xi[0] = -K * kw * p.r_mu_w;
     c.Xi[1] = -K * kn * p.r_mu_n;
     c.Xi[2] = -K * kg * p.r_mu_g;
     c.dXi_dSw[0] = -K * dkw * p.r_mu_w * p.r_den3;
     c.dXi_dSn[0] = 0.;
-    c.dXi_dSw[1] = 0.;
-    c.dXi_dSn[1] = -K * dkn * p.r_mu_n * p.r_den3;
+    c.dXi_dSw[1] = -K * dkn_w * p.r_mu_n * p.r_den3;
+    c.dXi_dSn[1] = -K * dkn_n * p.r_mu_n * p.r_den3;
     c.dXi_dSw[2] = K * dkg * p.r_mu_g * p.r_den3;
     c.dXi_dSn[2] = c.dXi_dSw[2];
     c.ro[0] = p.ro0_w * (1. + p.beta_w * (c.P[0] - p.P_atm));
@@ -105,7 +166,7 @@ CAPAZ_HD CoefD coef_d(const DevParams& p, const IO& a, int64_t l, bool inner) {
 }
 
 // Values only (neighbours in the gathers).
-template <class IO>
+template <int CL = -1, class IO>
 CAPAZ_HD Coef coef_v(const DevParams& p, const IO& a, int64_t l, bool inner) {
     Coef c;
     const double Pw = a.Pw[l], Sw = a.Sw[l], Sn = a.Sn[l], K = a.K[l];
@@ -113,10 +174,8 @@ CAPAZ_HD Coef coef_v(const DevParams& p, const IO& a, int64_t l, bool inner) {
     c.P[1] = inner ? Pw : a.Pn[l];
     c.P[2] = inner ? Pw : a.Pg[l];
     const double Swe = (Sw - p.S_wr) * p.r_den3, Sne = (Sn - p.S_nr) * p.r_den3;
-    double kw, kn, kg, d;
-    corey_d(p, Swe, kw, d);
-    corey_d(p, Sne, kn, d);
-    corey_d(p, 1. - Swe - Sne, kg, d);
+    double kw, kn, kg, d0, d1, d2, d3;
+    kr3_d<CL>(p, Swe, Sne, kw, kn, kg, d0, d1, d2, d3);
     c.Xi[0] = -K * kw * p.r_mu_w;
     c.Xi[1] = -K * kn * p.r_mu_n;
     c.Xi[2] = -K * kg * p.r_mu_g;
@@ -452,7 +511,9 @@ CAPAZ_HD void adj_coef_finish(const DevParams& p, const AdjIO& a, int64_t l, con
     a.g_lnK[l] += glnK;
 }
 
-// Pass 2 at cell (i, j, k), l its index, reading its neighbours from memory.
+// Pass 2 at cell (i, j, k), l its index, reading its neighbours from memory
+// (CL: as kr3_d).
+template <int CL = -1>
 CAPAZ_HD AdjNb adj_nb_at(const DevParams& p, const AdjIO& a, int i, int j, int k) {
     AdjNb n;
     n.exists = i >= 0 && i < p.Nx && j >= 0 && j < p.Ny && k >= 0 && k < p.Nz;
@@ -462,7 +523,7 @@ CAPAZ_HD AdjNb adj_nb_at(const DevParams& p, const AdjIO& a, int i, int j, int k
     if (!n.exists) return n;
     const int64_t q = (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz;
     n.interior = interior3(p, i, j, k);
-    n.c = coef_v(p, a, q, n.interior);
+    n.c = coef_v<CL>(p, a, q, n.interior);
     if (n.interior) {
         n.mu[0] = a.mu_w[q];
         n.mu[1] = a.mu_n[q];
@@ -471,20 +532,293 @@ CAPAZ_HD AdjNb adj_nb_at(const DevParams& p, const AdjIO& a, int i, int j, int k
     return n;
 }
 
+template <int CL = -1>
 CAPAZ_HD void adj_coef_cell(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l) {
     const bool inner = interior3(p, i, j, k);
-    const CoefD c = coef_d(p, a, l, inner);
+    const CoefD c = coef_d<CL>(p, a, l, inner);
     const double mu_self[3] = {inner ? a.mu_w[l] : 0., inner ? a.mu_n[l] : 0., inner ? a.mu_g[l] : 0.};
     double lP[3] = {0., 0., 0.}, lL[3] = {0., 0., 0.}, lro[3] = {0., 0., 0.};
     if (!(p.Nx < 2))
-        adj_coef_axis(p, c, mu_self, adj_nb_at(p, a, i - 1, j, k), adj_nb_at(p, a, i + 1, j, k), p.r_hx, false, lP,
+        adj_coef_axis(p, c, mu_self, adj_nb_at<CL>(p, a, i - 1, j, k), adj_nb_at<CL>(p, a, i + 1, j, k), p.r_hx, false, lP,
                       lL, lro);
-    adj_coef_axis(p, c, mu_self, adj_nb_at(p, a, i, j - 1, k), adj_nb_at(p, a, i, j + 1, k), p.r_hy, true, lP, lL,
+    adj_coef_axis(p, c, mu_self, adj_nb_at<CL>(p, a, i, j - 1, k), adj_nb_at<CL>(p, a, i, j + 1, k), p.r_hy, true, lP, lL,
                   lro);
     if (!(p.Nz < 2))
-        adj_coef_axis(p, c, mu_self, adj_nb_at(p, a, i, j, k - 1), adj_nb_at(p, a, i, j, k + 1), p.r_hz, false, lP,
+        adj_coef_axis(p, c, mu_self, adj_nb_at<CL>(p, a, i, j, k - 1), adj_nb_at<CL>(p, a, i, j, k + 1), p.r_hz, false, lP,
                       lL, lro);
     adj_coef_finish(p, a, l, c, lP, lL, lro);
+}
+
+// ---- two phases (two-phase.cpp with the Corey closure) ------------------------
+//
+// The state is (P_w, S_n) per cell; with no capillary pressure P_n = P_w in
+// every cell, so a cell's coefficients depend on its P_w (densities) and S_n
+// (k_r).  The step's Newton (newton2, two-phase.cpp:25-56) uses the true
+// Jacobian M = dF/dx of F = (ro_w(P)(1 - S_n) - A_w, ro_n(P) S_n - A_n), so
+// its backward iteration is the three-phase one with J = M.  Border
+// (two-phase.cpp:163-215): S_n from the inward cell (the injected value in
+// the top face's source window), P_w from the inward cell on the sides,
+// P_atm on the top, and on the bottom the fixed P_bottom or the inward cell's
+// new P_w plus (ro_n S_n + ro_w (1 - S_n)) m g h_y with this step's
+// densities of the inward cell's old state and its new S_n.
+// Field use in AdjIO: Pw, Sn (state), lPw1/lSn1, mu_w/mu_n, dPw/dSn, lPw/lSn.
+
+struct CoefD2 {
+    double P, Xi[2], ro[2];
+    double dXi_dSn[2], dro_dP[2];
+};
+
+template <bool N2 = false, class IO>
+CAPAZ_HD CoefD2 coef_d2(const DevParams& p, const IO& a, int64_t l) {
+    CoefD2 c;
+    const double Pw = a.Pw[l], Sn = a.Sn[l], K = a.K[l];
+    c.P = Pw;
+    const double se = (1. - Sn - p.S_wr) * p.r_den2c;  // dse/dS_n = -1/den2c
+    double kw, dkw, kn, dkn;
+    corey_d<N2>(p, se, kw, dkw);
+    corey_d<N2>(p, 1. - se, kn, dkn);
+    c.Xi[0] = -K * kw * p.r_mu_w;
+    c.Xi[1] = -K * kn * p.r_mu_n;
+    c.dXi_dSn[0] = K * dkw * p.r_mu_w * p.r_den2c;
+    c.dXi_dSn[1] = -K * dkn * p.r_mu_n * p.r_den2c;
+    c.ro[0] = p.ro0_w * (1. + p.beta_w * (Pw - p.P_atm));
+    c.ro[1] = p.ro0_n * (1. + p.beta_n * (Pw - p.P_atm));
+    c.dro_dP[0] = p.c_wb;
+    c.dro_dP[1] = p.c_nb;
+    return c;
+}
+
+// newton2's matrix (rows F1, F2; columns P, S_n) at (P, S_n).
+CAPAZ_HD void newton2_matrix(const DevParams& p, double P, double Sn, double* M) {
+    M[0] = p.c_wb * (1. - Sn);
+    M[1] = -p.ro0_w * (1. + p.beta_w * (P - p.P_atm));
+    M[2] = p.c_nb * Sn;
+    M[3] = p.ro0_n * (1. + p.beta_n * (P - p.P_atm));
+}
+
+// The forward iterates of newton2 from x0 = (P, S_n) with right-hand sides
+// A (xs[0..it]), solved by Cramer's rule.
+template <int IT>
+CAPAZ_HD int newton2_iterates(const DevParams& p, double P0, double Sn0, const double* A,
+                              double (&xs)[(IT > 0 ? IT : kMaxAdjIter) + 1][2]) {
+    constexpr int NX = IT > 0 ? IT : kMaxAdjIter;
+    const int it = IT > 0 ? IT : (p.iterations < kMaxAdjIter ? p.iterations : kMaxAdjIter);
+    xs[0][0] = P0;
+    xs[0][1] = Sn0;
+#if defined(__HIPCC__)
+#pragma unroll
+#endif
+    for (int k = 0; k < NX; ++k) {
+        if (k >= it) break;
+        const double P = xs[k][0], Sn = xs[k][1];
+        double M[4];
+        newton2_matrix(p, P, Sn, M);
+        const double F0 = -M[1] * (1. - Sn) - A[0], F1 = M[3] * Sn - A[1];
+        const double rd = 1. / (M[0] * M[3] - M[1] * M[2]);
+        xs[k + 1][0] = P - (M[3] * F0 - M[1] * F1) * rd;
+        xs[k + 1][1] = Sn - (M[0] * F1 - M[2] * F0) * rd;
+    }
+    return it;
+}
+
+// Backward newton2 over the iterates xs: lam (the adjoint of xs[it]) becomes
+// the adjoint of xs[0]; the adjoint of A is added to lA.  Per iteration,
+// w = M^-T lam, lA += w, lam_j -= w . (J[:, j] - (dM/dx_j) u) with J = M:
+//   j = P:   J = (c_wb (1 - S_n), c_nb S_n), (dM/dP) u = (-c_wb u1, c_nb u1)
+//   j = S_n: J = (-ro_w, ro_n),              (dM/dS_n) u = (-c_wb u0, c_nb u0)
+template <int IT>
+CAPAZ_HD void newton2_backward(const DevParams& p, const double (&xs)[(IT > 0 ? IT : kMaxAdjIter) + 1][2], int it,
+                               double* lam, double* lA) {
+    constexpr int NX = IT > 0 ? IT : kMaxAdjIter;
+#if defined(__HIPCC__)
+#pragma unroll
+#endif
+    for (int kk = NX - 1; kk >= 0; --kk) {
+        if (kk >= it) continue;
+        const double P = xs[kk][0], Sn = xs[kk][1];
+        const double u0 = P - xs[kk + 1][0], u1 = Sn - xs[kk + 1][1];
+        double M[4];
+        newton2_matrix(p, P, Sn, M);
+        // M^T w = lam: [[M0, M2], [M1, M3]] w = lam
+        const double rd = 1. / (M[0] * M[3] - M[1] * M[2]);
+        const double w0 = (M[3] * lam[0] - M[2] * lam[1]) * rd;
+        const double w1 = (M[0] * lam[1] - M[1] * lam[0]) * rd;
+        lA[0] += w0;
+        lA[1] += w1;
+        lam[0] -= w0 * (M[0] + p.c_wb * u1) + w1 * (M[2] - p.c_nb * u1);
+        lam[1] -= w0 * (M[1] + p.c_wb * u0) + w1 * (M[3] - p.c_nb * u0);
+    }
+}
+
+// Pass 1 of the two-phase adjoint at interior cell (i, j, k), l its index:
+// the step's densities ro and flux divergence div of the cell are known.
+// Folds the boundary cells' adjoints into the cell's new state (the transpose
+// of Border), runs newton2 backwards and writes mu, the direct adjoint d and
+// the porosity gradient.
+template <int IT>
+CAPAZ_HD void adj2_newton_core(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l, const double* ro,
+                               const double* div) {
+    const double Pw = a.Pw[l], Sn = a.Sn[l], dtm = a.dtm[l], m = a.m[l];
+    double A[2];
+    A[0] = ro[0] * (1. - Sn) - dtm * div[0];
+    A[1] = ro[1] * Sn - dtm * div[1];
+    double xs[(IT > 0 ? IT : kMaxAdjIter) + 1][2];
+    const int it = newton2_iterates<IT>(p, Pw, Sn, A, xs);
+    const double Sn1 = xs[it][1];
+    double lam[2] = {a.lPw1[l], a.lSn1[l]};
+    double dP_extra = 0., gm_extra = 0.;
+    const double ghy = p.g * p.hy;
+    for_boundary_of(p, i, j, k, [&](int bi, int bj, int bk) {
+        const int64_t lb = (int64_t)bi + (int64_t)bj * p.sy + (int64_t)bk * p.sz;
+        const double lbP = a.lPw1[lb], lbS = a.lSn1[lb];
+        if (!(bj == 0 && p.source > 0 && in_source_window(p, bi, bk))) lam[1] += lbS;
+        if (bj != 0 && bj != p.Ny - 1) {
+            lam[0] += lbP;
+        } else if (bj == p.Ny - 1 && !(p.P_bottom > 0)) {
+            // P_b = P_w(new) + (ro_n S_n(new) + ro_w (1 - S_n(new))) m g h_y
+            lam[0] += lbP;
+            lam[1] += lbP * (ro[1] - ro[0]) * m * ghy;
+            dP_extra += lbP * (p.c_nb * Sn1 + p.c_wb * (1. - Sn1)) * m * ghy;
+            gm_extra += lbP * (ro[1] * Sn1 + ro[0] * (1. - Sn1)) * ghy;
+        }
+    });
+    double lA[2] = {0., 0.};
+    newton2_backward<IT>(p, xs, it, lam, lA);
+    a.mu_w[l] = -dtm * lA[0];
+    a.mu_n[l] = -dtm * lA[1];
+    a.g_m[l] += (lA[0] * div[0] + lA[1] * div[1]) * dtm / m + gm_extra;
+    a.dPw[l] = lam[0] + lA[0] * p.c_wb * (1. - Sn) + lA[1] * p.c_nb * Sn + dP_extra;
+    a.dSn[l] = lam[1] - lA[0] * ro[0] + lA[1] * ro[1];
+}
+
+// Pass 1 from the flux divergence recorded by the forward step.
+template <int IT>
+CAPAZ_HD void adj2_newton_recorded(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l,
+                                   const double* div_rec, int64_t stride) {
+    const double Pw = a.Pw[l];
+    const double ro[2] = {ro_w_of(p, Pw), ro_n_of(p, Pw)};
+    const double div[2] = {div_rec[l], div_rec[stride + l]};
+    adj2_newton_core<IT>(p, a, i, j, k, l, ro, div);
+}
+
+// Pass 1 recomputing the cell's divergence with the step's own arithmetic
+// (the host build's path).
+template <int IT>
+CAPAZ_HD void adj2_newton_cell(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l) {
+    StepIO io{};
+    io.Pw = a.Pw;
+    io.Sn = a.Sn;
+    io.m = a.m;
+    io.K = a.K;
+    const Coef c = cell_coef(p, io, l, true);
+    auto nb = [&](int ii, int jj, int kk) {
+        const int64_t q = (int64_t)ii + (int64_t)jj * p.sy + (int64_t)kk * p.sz;
+        return cell_coef(p, io, q, cell_inner(p, ii, jj, kk));
+    };
+    Coef xm = c, xp = c, zm = c, zp = c;
+    if (!(p.Nx < 2)) {
+        xm = nb(i - 1, j, k);
+        xp = nb(i + 1, j, k);
+    }
+    if (!(p.Nz < 2)) {
+        zm = nb(i, j, k - 1);
+        zp = nb(i, j, k + 1);
+    }
+    const Coef ym = nb(i, j - 1, k), yp = nb(i, j + 1, k);
+    const double div[2] = {coef_divergence(p, 0, c, xm, xp, ym, yp, zm, zp),
+                           coef_divergence(p, 1, c, xm, xp, ym, yp, zm, zp)};
+    adj2_newton_core<IT>(p, a, i, j, k, l, c.ro, div);
+}
+
+// Pass 2 of the two-phase adjoint at cell (i, j, k), l its index (every
+// cell): the transposed stencil of the fluxes that read the cell's
+// coefficients (its own, if interior, and its interior neighbours'), chained
+// to (P_w, S_n) and ln K, plus pass 1's direct share if interior.
+struct AdjNb2 {
+    double P, Xi[2], ro[2], mu[2];
+    bool exists, interior;
+};
+
+template <bool N2 = false>
+CAPAZ_HD AdjNb2 adj2_nb_at(const DevParams& p, const AdjIO& a, int i, int j, int k) {
+    AdjNb2 n{};
+    n.exists = i >= 0 && i < p.Nx && j >= 0 && j < p.Ny && k >= 0 && k < p.Nz;
+    if (!n.exists) return n;
+    const int64_t q = (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz;
+    n.interior = interior_newton2(p, i, j, k);
+    const CoefD2 c = coef_d2<N2>(p, a, q);
+    n.P = c.P;
+    n.Xi[0] = c.Xi[0];
+    n.Xi[1] = c.Xi[1];
+    n.ro[0] = c.ro[0];
+    n.ro[1] = c.ro[1];
+    if (n.interior) {
+        n.mu[0] = a.mu_w[q];
+        n.mu[1] = a.mu_n[q];
+    }
+    return n;
+}
+
+CAPAZ_HD void adj2_coef_axis(const DevParams& p, const CoefD2& c, bool inner, const double* mu_self, const AdjNb2& m,
+                             const AdjNb2& q, double rh, bool grav, double* lP, double* lL, double* lro) {
+    for (int ph = 0; ph < 2; ++ph) {
+        const double Lc = -c.Xi[ph] * c.ro[ph];
+        if (inner) {
+            const double gr = grav ? p.g * c.ro[ph] : 0.;
+            const double x1 = -((c.P - m.P) * rh) + gr, x2 = -((q.P - c.P) * rh) + gr;
+            const DirD d = directed_d(x1, x2, Lc, -m.Xi[ph] * m.ro[ph], -q.Xi[ph] * q.ro[ph], rh);
+            lP[ph] += mu_self[ph] * (d.dx2 - d.dx1) * rh;
+            lL[ph] += mu_self[ph] * d.dLc;
+            if (grav) lro[ph] += mu_self[ph] * (d.dx1 + d.dx2) * p.g;
+        }
+        if (q.exists && q.interior) {
+            const double gr = grav ? p.g * q.ro[ph] : 0.;
+            const double x1 = -((q.P - c.P) * rh) + gr;
+            const double Lq = -q.Xi[ph] * q.ro[ph];
+            const double w1 = upwind_w(x1);
+            lP[ph] += q.mu[ph] * ((-(1. - w1) * Lq - w1 * Lc) * rh) * rh;
+            lL[ph] += q.mu[ph] * (-(x1 > 0. ? x1 : 0.) * rh);
+        }
+        if (m.exists && m.interior) {
+            const double gr = grav ? p.g * m.ro[ph] : 0.;
+            const double x2 = -((c.P - m.P) * rh) + gr;
+            const double Lq = -m.Xi[ph] * m.ro[ph];
+            const double w2 = upwind_w(x2);
+            lP[ph] += m.mu[ph] * (-((w2 * Lq + (1. - w2) * Lc) * rh) * rh);
+            lL[ph] += m.mu[ph] * ((x2 < 0. ? x2 : 0.) * rh);
+        }
+    }
+}
+
+template <bool N2 = false>
+CAPAZ_HD void adj2_coef_cell(const DevParams& p, const AdjIO& a, int i, int j, int k, int64_t l) {
+    const bool inner = interior_newton2(p, i, j, k);
+    const CoefD2 c = coef_d2<N2>(p, a, l);
+    const double mu_self[2] = {inner ? a.mu_w[l] : 0., inner ? a.mu_n[l] : 0.};
+    double lP[2] = {0., 0.}, lL[2] = {0., 0.}, lro[2] = {0., 0.};
+    if (!(p.Nx < 2))
+        adj2_coef_axis(p, c, inner, mu_self, adj2_nb_at<N2>(p, a, i - 1, j, k), adj2_nb_at<N2>(p, a, i + 1, j, k), p.r_hx,
+                       false, lP, lL, lro);
+    adj2_coef_axis(p, c, inner, mu_self, adj2_nb_at<N2>(p, a, i, j - 1, k), adj2_nb_at<N2>(p, a, i, j + 1, k), p.r_hy, true,
+                   lP, lL, lro);
+    if (!(p.Nz < 2))
+        adj2_coef_axis(p, c, inner, mu_self, adj2_nb_at<N2>(p, a, i, j, k - 1), adj2_nb_at<N2>(p, a, i, j, k + 1), p.r_hz,
+                       false, lP, lL, lro);
+    double lPw = 0., lSn = 0., glnK = 0.;
+    for (int ph = 0; ph < 2; ++ph) {
+        const double lXi = -lL[ph] * c.ro[ph];
+        const double lr = lro[ph] - lL[ph] * c.Xi[ph];
+        lPw += lP[ph] + lr * c.dro_dP[ph];
+        lSn += lXi * c.dXi_dSn[ph];
+        glnK += lXi * c.Xi[ph];
+    }
+    if (inner) {
+        lPw += a.dPw[l];
+        lSn += a.dSn[l];
+    }
+    a.lPw[l] = lPw;
+    a.lSn[l] = lSn;
+    a.g_lnK[l] += glnK;
 }
 
 // ---- tangent-linear step -------------------------------------------------------

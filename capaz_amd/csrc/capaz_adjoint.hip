@@ -38,13 +38,37 @@ __global__ void __launch_bounds__(256) k_adj_newton_rec(DevParams p, AdjIO a, co
 // The grid is a multiple of the 8 XCDs; each XCD takes a contiguous run of
 // cells (workgroups go to the XCDs round robin), so the z neighbours a cell
 // gathers from were read by the same XCD and hit its L2.
+// CL: the closure (1 Corey, 0 the reference's van Genuchten).
+template <int CL>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_adj_coef(DevParams p, AdjIO a) {
     const uint32_t b = (blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u;
     const uint32_t t = b * 256u + threadIdx.x;
     if (t >= (uint32_t)p.n) return;
     int i, j, k;
     box_cell(t, (uint32_t)p.Nx, (uint32_t)p.Ny, 0, 0, 0, i, j, k);
-    adj_coef_cell(p, a, i, j, k, (int64_t)t);
+    adj_coef_cell<CL>(p, a, i, j, k, (int64_t)t);
+}
+
+// Two phases: pass 1 over the interior cells (dense numbering), pass 2 over
+// every cell (XCD-contiguous runs, as k_adj_coef).
+template <int IT>
+__global__ void __launch_bounds__(256) k_adj2_newton_rec(DevParams p, AdjIO a, const double* div, int64_t stride,
+                                                         uint32_t ni, uint32_t nj, uint32_t total) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= total) return;
+    int i, j, k;
+    box_cell(t, ni, nj, 1, 1, p.Nz >= 3 ? 1 : 0, i, j, k);
+    adj2_newton_recorded<IT>(p, a, i, j, k, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz, div, stride);
+}
+
+template <bool N2>
+__global__ void __launch_bounds__(256) k_adj2_coef(DevParams p, AdjIO a) {
+    const uint32_t b = (blockIdx.x % 8u) * (gridDim.x / 8u) + blockIdx.x / 8u;
+    const uint32_t t = b * 256u + threadIdx.x;
+    if (t >= (uint32_t)p.n) return;
+    int i, j, k;
+    box_cell(t, (uint32_t)p.Nx, (uint32_t)p.Ny, 0, 0, 0, i, j, k);
+    adj2_coef_cell<N2>(p, a, i, j, k, (int64_t)t);
 }
 
 // Tangent-linear step: interior cells (dense numbering), then the faces.
@@ -62,23 +86,30 @@ __global__ void __launch_bounds__(256) k_tan_faces(DevParams p, TanIO a) {
     tan_boundary(p, a, i, j, k);
 }
 
-// out_w[c] = S_w[cells[c]], out_n[c] = S_n[cells[c]]
+// out_w[c] = S_w[cells[c]], out_n[c] = S_n[cells[c]]; Sw == NULL (two
+// phases): S_w = 1 - S_n.
 __global__ void k_obs_gather(const double* Sw, const double* Sn, const int64_t* cells, int64_t ncells, double* out_w,
                              double* out_n) {
     const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (c >= ncells) return;
-    out_w[c] = Sw[cells[c]];
-    out_n[c] = Sn[cells[c]];
+    const double sn = Sn[cells[c]];
+    out_w[c] = Sw ? Sw[cells[c]] : 1. - sn;
+    out_n[c] = sn;
 }
 
-// lam_S[cells[c]] += seen[c] - want[c] (cells distinct)
+// lam_S[cells[c]] += seen[c] - want[c] (cells distinct); lSw == NULL (two
+// phases, S_w = 1 - S_n): the S_w mismatch goes to lam_S_n with a minus sign.
 __global__ void k_obs_adjoint(double* lSw, double* lSn, const int64_t* cells, int64_t ncells, const double* seen_w,
                               const double* seen_n, const double* want_w, const double* want_n) {
     const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (c >= ncells) return;
     const int64_t l = cells[c];
-    lSw[l] += seen_w[c] - want_w[c];
-    lSn[l] += seen_n[c] - want_n[c];
+    if (lSw) {
+        lSw[l] += seen_w[c] - want_w[c];
+        lSn[l] += seen_n[c] - want_n[c];
+    } else {
+        lSn[l] += (seen_n[c] - want_n[c]) - (seen_w[c] - want_w[c]);
+    }
 }
 
 // ---- tiled passes: the z-marching tiles of the fused step -------------------
@@ -219,13 +250,15 @@ hipError_t launch_adjoint_step_tiled(const DevParams& p, const AdjIO& a, hipStre
     const int64_t txy = (int64_t)((nx + kTX - 1) / kTX) * ((ny + kTY - 1) / kTY);
     const int kc1 = adj_kchunk(nz, txy);
     const dim3 g1((unsigned)((nx + kTX - 1) / kTX), (unsigned)((ny + kTY - 1) / kTY), (unsigned)((nz + kc1 - 1) / kc1));
-    const bool corey2 = p.corey_n == 2.;
+    const int closure = p.closure == 0 ? 0 : (p.corey_n == 2. ? 2 : 1);
 #define CAPAZ_ADJ1(IT)                                                                                         \
     do {                                                                                                       \
-        if (corey2)                                                                                            \
+        if (closure == 2)                                                                                      \
             hipLaunchKernelGGL((k_adj_newton_z<2, IT>), g1, dim3(kTX, kTY), 0, s, p, io, a, kc1);             \
-        else                                                                                                   \
+        else if (closure == 1)                                                                                 \
             hipLaunchKernelGGL((k_adj_newton_z<1, IT>), g1, dim3(kTX, kTY), 0, s, p, io, a, kc1);             \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_adj_newton_z<0, IT>), g1, dim3(kTX, kTY), 0, s, p, io, a, kc1);             \
     } while (0)
     switch (p.iterations) {
         case 1: CAPAZ_ADJ1(1); break;
@@ -246,7 +279,13 @@ hipError_t launch_adjoint_step_tiled(const DevParams& p, const AdjIO& a, hipStre
 }
 
 hipError_t launch_adjoint_coef(const DevParams& p, const AdjIO& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_coef, dim3((unsigned)((p.n + 255) / 256 + 7) / 8 * 8), dim3(256), 0, s, p, a);
+    const dim3 grid((unsigned)((p.n + 255) / 256 + 7) / 8 * 8);
+    if (p.closure == 0)
+        hipLaunchKernelGGL(k_adj_coef<0>, grid, dim3(256), 0, s, p, a);
+    else if (p.corey_n == 2.)
+        hipLaunchKernelGGL(k_adj_coef<2>, grid, dim3(256), 0, s, p, a);
+    else
+        hipLaunchKernelGGL(k_adj_coef<1>, grid, dim3(256), 0, s, p, a);
     return hipGetLastError();
 }
 
@@ -254,6 +293,18 @@ hipError_t launch_adjoint_step_recorded(const DevParams& p, const AdjIO& a, cons
                                         hipStream_t s) {
     const uint32_t ni = p.Nx >= 3 ? p.Nx - 2 : 1, nj = p.Ny - 2, nk = p.Nz >= 3 ? p.Nz - 2 : 1;
     const uint32_t total = ni * nj * nk;
+    if (p.model == 2) {
+        hipLaunchKernelGGL(k_adj2_newton_rec<0>, dim3((total + 255) / 256), dim3(256), 0, s, p, a, div, stride, ni, nj,
+                           total);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        const dim3 grid((unsigned)((p.n + 255) / 256 + 7) / 8 * 8);
+        if (p.corey_n == 2.)
+            hipLaunchKernelGGL(k_adj2_coef<true>, grid, dim3(256), 0, s, p, a);
+        else
+            hipLaunchKernelGGL(k_adj2_coef<false>, grid, dim3(256), 0, s, p, a);
+        return hipGetLastError();
+    }
     // the run-time iteration count: its loop holds fewer live registers
     // (144 VGPRs) than the fully unrolled fixed counts, without scratch
     hipLaunchKernelGGL(k_adj_newton_rec<0>, dim3((total + 255) / 256), dim3(256), 0, s, p, a, div, stride, ni, nj,

@@ -41,8 +41,11 @@ struct Obs {
 };
 
 int check_model(capaz_ctx* c) {
-    if (c->user.model != 3 || c->user.energy || c->user.closure != CAPAZ_CLOSURE_COREY)
-        return fail(c, CAPAZ_ERR_INVALID, "adjoint: three-phase model with the Corey closure only");
+    if (c->user.energy) return fail(c, CAPAZ_ERR_INVALID, "adjoint: models without the energy equation only");
+    if (c->user.model == 2 && c->user.closure != CAPAZ_CLOSURE_COREY)
+        return fail(c, CAPAZ_ERR_INVALID, "adjoint: two phases with the Corey closure only");
+    if (c->user.model == 2 && c->tuning.adjoint_kernel != 2)
+        return fail(c, CAPAZ_ERR_INVALID, "adjoint: two phases with adjoint_kernel 2 only");
     if (c->p.iterations > capaz::kMaxAdjIter)
         return fail(c, CAPAZ_ERR_INVALID, "adjoint: at most 8 Newton iterations");
     if (c->p.Ny > 65535 || c->p.Nz > 65535) return fail(c, CAPAZ_ERR_INVALID, "adjoint: Ny, Nz <= 65535");
@@ -115,10 +118,16 @@ int fused_between(capaz_ctx* c, hipStream_t s, const double* from, double* to, s
     return CAPAZ_OK;
 }
 
+// The state fields the model keeps between steps: all five for three phases,
+// P_w and S_n for two (their slots keep the five-field layout).
+bool state_field(const capaz_ctx* c, int f) { return c->p.model == 3 || f == 0 || f == 2; }
+
 int copy_state(capaz_ctx* c, double* dst, size_t stride) {
     const size_t bytes = (size_t)c->p.n * sizeof(double);
     for (int f = 0; f < kState; ++f)
-        ADJ_HIP(c, hipMemcpyAsync(dst + f * stride, c->cur[kStateField[f]], bytes, hipMemcpyDeviceToDevice, c->stream));
+        if (state_field(c, f))
+            ADJ_HIP(c, hipMemcpyAsync(dst + f * stride, c->cur[kStateField[f]], bytes, hipMemcpyDeviceToDevice,
+                                      c->stream));
     return CAPAZ_OK;
 }
 
@@ -148,7 +157,8 @@ int forward(capaz_ctx* c, long nsteps, const Obs& ob, const int64_t* d_cells, do
         if (every > 0 && t1 % every == 0 && t1 < nsteps &&
             (rc = copy_state(c, ckpt + (size_t)(t1 / every) * kState * stride, stride)))
             return rc;
-        if ((rc = gather(t1, c->cur[capaz::F_S_w], c->cur[capaz::F_S_n]))) return rc;
+        // two phases: S_w = 1 - S_n (the S_w array is an intermediate)
+        if ((rc = gather(t1, c->p.model == 3 ? c->cur[capaz::F_S_w] : nullptr, c->cur[capaz::F_S_n]))) return rc;
     }
     if (!rec_seg) return CAPAZ_OK;
     // the recorded tail starts from the checkpoint just taken (or the initial copy)
@@ -160,10 +170,12 @@ int forward(capaz_ctx* c, long nsteps, const Obs& ob, const int64_t* d_cells, do
         if ((rc = fused_between(c, c->stream, slot(t), slot(t + 1), stride,
                                 rec_div + (size_t)(t - rec_from) * 3 * stride)))
             return rc;
-        if ((rc = gather(t + 1, slot(t + 1) + stride, slot(t + 1) + 2 * stride))) return rc;
+        if ((rc = gather(t + 1, c->p.model == 3 ? slot(t + 1) + stride : nullptr, slot(t + 1) + 2 * stride)))
+            return rc;
     }
     const size_t bytes = (size_t)c->p.n * sizeof(double);
     for (int f = 0; f < kState; ++f) {
+        if (!state_field(c, f)) continue;
         ADJ_HIP(c, hipMemcpyAsync(c->cur[kStateField[f]], slot(nsteps) + f * stride, bytes, hipMemcpyDeviceToDevice,
                                   c->stream));
         ADJ_HIP(c, hipMemcpyAsync(c->alt[kStateField[f]], slot(nsteps - 1) + f * stride, bytes,
@@ -222,6 +234,7 @@ int capaz_tangent(capaz_ctx* c, long nsteps, const capaz_observations* o, const 
     if (!c || !J || !dJ) return CAPAZ_ERR_INVALID;
     if ((rc = detail::check_ctx(c))) return rc;
     if ((rc = check_model(c))) return rc;
+    if (c->p.model != 3) return fail(c, CAPAZ_ERR_INVALID, "tangent: the three-phase model only");
     Obs ob;
     if ((rc = check_obs(c, nsteps, o, &ob))) return rc;
     if ((rc = detail::scan_m(c))) return rc;
@@ -388,8 +401,9 @@ int capaz_adjoint(capaz_ctx* c, long nsteps, const capaz_observations* o, int ch
         const int s = ob.sample_of[(size_t)step];
         if (s < 0) return CAPAZ_OK;
         const size_t off = (size_t)s * 2 * ob.ncells;
-        hipError_t e = launch_obs_adjoint(l + 1 * S, l + 2 * S, d_cells, ob.ncells, seen + off, seen + off + ob.ncells,
-                                          d_want + off, d_want + off + ob.ncells, c->stream);
+        hipError_t e = launch_obs_adjoint(c->p.model == 3 ? l + 1 * S : nullptr, l + 2 * S, d_cells, ob.ncells,
+                                          seen + off, seen + off + ob.ncells, d_want + off, d_want + off + ob.ncells,
+                                          c->stream);
         return e == hipSuccess ? CAPAZ_OK : hip_fail(c, e, "launch_obs_adjoint");
     };
     // reverse sweep

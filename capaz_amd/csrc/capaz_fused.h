@@ -403,11 +403,16 @@ struct FusedStore {
     }
 };
 
+// Two phases: boundary cells included (kFusedFaces); three phases: followed
+// by k_fused_faces.
+template <int MODEL>
+constexpr bool kFusedFaces = MODEL == 2;
+
 // FusedStore that also records the cell's flux divergence div_a (field a at
 // div + a*stride), for the adjoint's backward Newton.
 template <int MODEL>
 struct FusedStoreRecord {
-    FusedStore<MODEL> step;
+    FusedStore<MODEL, kFusedFaces<MODEL>> step;
     double* div;
     int64_t stride;
     __device__ __forceinline__ void operator()(const DevParams& p, int i, int j, int k, int64_t l,
@@ -594,13 +599,10 @@ __global__ void __launch_bounds__(TX * TY) k_fused_z_record(DevParams p, StepIO 
     int bx, by, bz;
     xcd_tile(bx, by, bz);
     fused_z_tile<MODEL, CLOSURE, TX, TY, CARRY>(p, io, kchunk, bx, by, bz, lds,
-                                                FusedStoreRecord<MODEL>{FusedStore<MODEL>{io}, div, stride});
+                                                FusedStoreRecord<MODEL>{FusedStore<MODEL, kFusedFaces<MODEL>>{io},
+                                                                        div, stride});
 }
 
-// Two phases: boundary cells included (kFusedFaces); three phases: followed
-// by k_fused_faces.
-template <int MODEL>
-constexpr bool kFusedFaces = MODEL == 2;
 
 template <int MODEL, int CLOSURE, int TX, int TY, bool CARRY>
 __global__ void __launch_bounds__(TX * TY) k_fused_z(DevParams p, StepIO io, int kchunk) {

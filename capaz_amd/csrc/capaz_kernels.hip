@@ -325,11 +325,29 @@ hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, h
 
 hipError_t launch_fused_record(const DevParams& p, const StepIO& io, int kchunk, double* div, int64_t stride,
                                hipStream_t s) {
-    if (p.model != 3 || p.closure == 0) return hipErrorInvalidValue;
     const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
+    if (p.model == 2) {
+        // the Corey closures; the tiles write the boundary cells
+        if (p.closure == 0) return hipErrorInvalidValue;
+        kchunk = fused_kchunk(p, kchunk, 8, resident_groups<k_fused_z_record<2, 2, 64, 8, true>>(512));
+        dim3 grid2((nx + 63) / 64, (ny + 7) / 8, (nz + kchunk - 1) / kchunk);
+        if (p.corey_n == 2.)
+            hipLaunchKernelGGL((k_fused_z_record<2, 2, 64, 8, true>), grid2, dim3(64, 8), 0, s, p, io, kchunk, div,
+                               stride);
+        else
+            hipLaunchKernelGGL((k_fused_z_record<2, 1, 64, 8, true>), grid2, dim3(64, 8), 0, s, p, io, kchunk, div,
+                               stride);
+        return hipGetLastError();
+    }
+    if (p.model != 3) return hipErrorInvalidValue;
     kchunk = fused_kchunk(p, kchunk, 8, resident_groups<k_fused_z_record<3, 2, 64, 8, true>>(512));
     dim3 grid((nx + 63) / 64, (ny + 7) / 8, (nz + kchunk - 1) / kchunk);
-    if (p.corey_n == 2.) {
+    if (p.closure == 0) {
+        hipLaunchKernelGGL((k_fused_z_record<3, 0, 64, 8, true>), grid, dim3(64, 8), 0, s, p, io, kchunk, div, stride);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_fused_faces<3, 0>), grid_for(fused_face_cells(p)), dim3(256), 0, s, p, io);
+    } else if (p.corey_n == 2.) {
         hipLaunchKernelGGL((k_fused_z_record<3, 2, 64, 8, true>), grid, dim3(64, 8), 0, s, p, io, kchunk, div, stride);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

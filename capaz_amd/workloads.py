@@ -143,9 +143,11 @@ def ensemble(count: int = 64, seed: int = 1003, dt: float = 1e-6):
     return keys, out
 
 
-def history_matching(n: int = 96, nsteps: int = 400, seed: int = 1004, columns: int = 16, every: int = 50):
+def history_matching(n: int = 96, nsteps: int = 400, seed: int = 1004, columns: int = 16, every: int = 50,
+                     phases: int = 3):
     """Permeability inversion on an n^3 three-phase grid (physics of
-    three_phase_injection): a seeded "true" ln K (ln-sigma 1.0 around
+    three_phase_injection; phases=2: of buckley_leverett_2ph, with the same
+    dt): a seeded "true" ln K (ln-sigma 1.0 around
     ln 6.64e-11, 6-cell Gaussian correlation), the trial ln K = its mean
     everywhere, porosity 0.4(1 + 0.05 N(0,1)) clipped to [0.3, 0.5] for both,
     and `columns` seeded observation columns (i, k) whose interior cells
@@ -157,7 +159,7 @@ def history_matching(n: int = 96, nsteps: int = 400, seed: int = 1004, columns: 
     Returns (keys, state, lnK_true, lnK_trial, obs) with state the initial
     primary fields and porosity, obs = {cells, steps} (the observed values
     come from a forward run with lnK_true)."""
-    keys, st = three_phase_injection(n, seed=seed)
+    keys, st = three_phase_injection(n, seed=seed) if phases == 3 else buckley_leverett_2ph(n)
     keys = dict(keys, DT=1e-6)
     rng = np.random.default_rng(seed + 1)
     lnK_true = _correlated_lnK(rng, n, 6.0, 1.0)

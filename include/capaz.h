@@ -309,8 +309,11 @@ int capaz_write_plot_rows(const char* path, const double* table, long nrows, int
  * the context's current state is
  *     J = 0.5 * sum_s sum_c [(S_w(c, steps[s]) - S_w_obs[s][c])^2
  *                            + (S_n(c, steps[s]) - S_n_obs[s][c])^2]
- * over the observed cells c and steps.  Three-phase model, Corey closure, at
- * most 8 Newton iterations, no m == 0 cells. */
+ * over the observed cells c and steps (two phases: S_w = 1 - S_n).  The
+ * three-phase model with the Corey or the reference's van Genuchten closures,
+ * or the two-phase model with the Corey closure (capaz_adjoint with
+ * adjoint_kernel 2), no energy equation, at most 8 Newton iterations, no
+ * m == 0 cells.  capaz_tangent: three phases. */
 typedef struct capaz_observations {
     size_t ncells;
     const int64_t* cells;  /* distinct linear cell indices i + j*Nx + k*Nx*Ny */

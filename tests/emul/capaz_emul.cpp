@@ -86,6 +86,18 @@ int capaz_emul_adjoint_step(const capaz_params* u, double** state, double** lam1
     a.mu_w = work[0]; a.mu_n = work[1]; a.mu_g = work[2]; a.dPw = work[3]; a.dSw = work[4]; a.dSn = work[5];
     a.lPw = lam[0]; a.lSw = lam[1]; a.lSn = lam[2]; a.lPn = lam[3]; a.lPg = lam[4];
     a.g_lnK = grad[0]; a.g_m = grad[1];
+    if (p.model == 2) {  // two phases: state (P_w, S_n); the other slots are unused
+        for (int k = 0; k < p.Nz; ++k)
+            for (int j = 0; j < p.Ny; ++j)
+                for (int i = 0; i < p.Nx; ++i)
+                    if (capaz::interior_newton2(p, i, j, k))
+                        capaz::adj2_newton_cell<0>(p, a, i, j, k, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz);
+        for (int k = 0; k < p.Nz; ++k)
+            for (int j = 0; j < p.Ny; ++j)
+                for (int i = 0; i < p.Nx; ++i)
+                    capaz::adj2_coef_cell(p, a, i, j, k, (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz);
+        return 0;
+    }
     for (int k = 0; k < p.Nz; ++k)
         for (int j = 0; j < p.Ny; ++j)
             for (int i = 0; i < p.Nx; ++i)
@@ -119,6 +131,31 @@ int capaz_emul_tangent_step(const capaz_params* u, double** state, double** dsta
         for (int j = 0; j < p.Ny; ++j)
             for (int i = 0; i < p.Nx; ++i)
                 if (!capaz::interior3(p, i, j, k)) capaz::tan_boundary(p, a, i, j, k);
+    return 0;
+}
+
+// vg_d (capaz_adjoint.h) at effective saturations (x, y): k = {k_w, k_n, k_g},
+// d = {dk_w/dx, dk_n/dx, dk_n/dy, dk_g/dz}.
+int capaz_emul_vg_d(const capaz_params* u, double x, double y, double* k, double* d) {
+    if (!capaz::validate(*u).empty()) return -1;
+    const capaz::DevParams p = capaz::derive(*u);
+    capaz::vg_d(p, x, y, k[0], k[1], k[2], d[0], d[1], d[2], d[3]);
+    return 0;
+}
+
+// coef_d (capaz_adjoint.h) of one cell state x = {Pw, Sw, Sn, Pn, Pg, K}:
+// out = {Xi[3], dXi_dSw[3], dXi_dSn[3]}.
+int capaz_emul_coef_d(const capaz_params* u, const double* x, int inner, double* out) {
+    if (!capaz::validate(*u).empty()) return -1;
+    const capaz::DevParams p = capaz::derive(*u);
+    capaz::AdjIO a{};
+    a.Pw = x; a.Sw = x + 1; a.Sn = x + 2; a.Pn = x + 3; a.Pg = x + 4; a.K = x + 5;
+    const capaz::CoefD c = capaz::coef_d(p, a, 0, inner != 0);
+    for (int ph = 0; ph < 3; ++ph) {
+        out[ph] = c.Xi[ph];
+        out[3 + ph] = c.dXi_dSw[ph];
+        out[6 + ph] = c.dXi_dSn[ph];
+    }
     return 0;
 }
 

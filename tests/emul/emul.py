@@ -37,6 +37,10 @@ def lib():
         L.capaz_emul_reverse_matrix.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_int]
         L.capaz_emul_adjoint_step.argtypes = [ctypes.POINTER(Params), PP, PP, PP, PP, PP]
         L.capaz_emul_tangent_step.argtypes = [ctypes.POINTER(Params), PP, PP, PP, PP]
+        L.capaz_emul_vg_d.argtypes = [ctypes.POINTER(Params), ctypes.c_double, ctypes.c_double,
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.capaz_emul_coef_d.argtypes = [ctypes.POINTER(Params), ctypes.POINTER(ctypes.c_double), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double)]
         L.capaz_emul_validate.argtypes = [ctypes.POINTER(Params)]
         L.capaz_emul_validate.restype = ctypes.c_char_p
         _lib = L
@@ -46,6 +50,28 @@ def lib():
 def validate(keys: dict) -> str:
     p = params_from_keys(keys)
     return lib().capaz_emul_validate(ctypes.byref(p)).decode()
+
+
+def vg_d(keys: dict, x: float, y: float):
+    """The adjoint's van Genuchten k_r at effective saturations (x, y):
+    ((k_w, k_n, k_g), (dk_w/dx, dk_n/dx, dk_n/dy, dk_g/dz))."""
+    k, d = (ctypes.c_double * 3)(), (ctypes.c_double * 4)()
+    p = params_from_keys(keys)
+    if lib().capaz_emul_vg_d(ctypes.byref(p), x, y, k, d):
+        raise ValueError("invalid parameters")
+    return tuple(k), tuple(d)
+
+
+def coef_d(keys: dict, Pw: float, Sw: float, Sn: float, K: float):
+    """The adjoint's coefficients of one interior cell: (Xi[3], dXi/dS_w[3],
+    dXi/dS_n[3])."""
+    x = (ctypes.c_double * 6)(Pw, Sw, Sn, Pw, Pw, K)
+    out = (ctypes.c_double * 9)()
+    p = params_from_keys(keys)
+    if lib().capaz_emul_coef_d(ctypes.byref(p), x, 1, out):
+        raise ValueError("invalid parameters")
+    v = list(out)
+    return v[0:3], v[3:6], v[6:9]
 
 
 def reverse_matrix(a: np.ndarray, fast: bool = False, a02zero: bool = False) -> tuple[int, np.ndarray]:
@@ -177,12 +203,15 @@ def gradient(keys: dict, fields: dict, nsteps: int, obs: dict):
     """J and dJ/d lnK, dJ/dm of `nsteps` fused steps from `fields`, by the
     host build of the forward step and of the adjoint (all states kept)."""
     d = EmulDomain(keys, fields)
+    two = d.p.model == 2  # state (P_w, S_n); S_w = 1 - S_n
     states, seen = [], {}
     for t in range(nsteps):
-        states.append({n: d.arrays[n].copy() for n in ADJ_STATE + ("m", "K")})
+        states.append({n: d.arrays[n].copy() if n in d.arrays else np.zeros(d.shape)
+                       for n in ADJ_STATE + ("m", "K")})
         d.fused_step()
         if t + 1 in obs["steps"]:
-            seen[t + 1] = (d.arrays["S_w"].ravel().copy(), d.arrays["S_n"].ravel().copy())
+            Sn = d.arrays["S_n"].ravel().copy()
+            seen[t + 1] = (1. - Sn if two else d.arrays["S_w"].ravel().copy(), Sn)
     J = objective(seen, obs)
     shape = d.shape
     grad = {"lnK": np.zeros(shape), "m": np.zeros(shape)}
@@ -192,8 +221,13 @@ def gradient(keys: dict, fields: dict, nsteps: int, obs: dict):
             s_i = list(obs["steps"]).index(step)
             Sw, Sn = seen[step]
             lw, ln = lam["S_w"].reshape(-1), lam["S_n"].reshape(-1)
-            np.add.at(lw, obs["cells"], Sw[obs["cells"]] - obs["S_w"][s_i])
-            np.add.at(ln, obs["cells"], Sn[obs["cells"]] - obs["S_n"][s_i])
+            dw = Sw[obs["cells"]] - obs["S_w"][s_i]
+            dn = Sn[obs["cells"]] - obs["S_n"][s_i]
+            if two:
+                np.add.at(ln, obs["cells"], dn - dw)
+            else:
+                np.add.at(lw, obs["cells"], dw)
+                np.add.at(ln, obs["cells"], dn)
 
     lam = {n: np.zeros(shape) for n in ADJ_STATE}
     add_obs(lam, nsteps)

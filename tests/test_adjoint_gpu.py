@@ -2,9 +2,10 @@
 capaz_adjoint).
 
   * J from capaz_objective and from capaz_adjoint equal the host build's J
-    bit for bit (the forward steps are the oracle's bits in the Corey
-    closure), and the gradient matches the host build of the same adjoint
-    bodies to rounding;
+    bit for bit in the Corey closure (the forward steps are the oracle's
+    bits; with the reference's van Genuchten closures the device pow gives
+    J to 1e-12), and the gradient matches the host build of the same adjoint
+    bodies to rounding, in both closures;
   * the gradient does not depend on the checkpoint interval (each segment
     is recomputed with the same bits) nor on whether the recompute runs on a
     second stream, and the default interval records the whole trajectory
@@ -27,8 +28,8 @@ pytestmark = pytest.mark.gpu
 STATE = ("P_w", "S_w", "S_n", "P_n", "P_g")
 
 
-def problem(nx, ny, nz, seed, nsteps):
-    keys = case("3ph", nx, ny, nz, CLOSURE="corey", S_WR_0=0.1, S_NR_0=0.05, SOURCE=1, DT=1e-4, ITERATIONS=4)
+def problem(nx, ny, nz, seed, nsteps, closure="corey"):
+    keys = case("3ph", nx, ny, nz, CLOSURE=closure, S_WR_0=0.1, S_NR_0=0.05, SOURCE=1, DT=1e-4, ITERATIONS=4)
     st = random_state(keys, seed)
     rng = np.random.default_rng(seed + 1)
     shape = st["P_w"].shape
@@ -45,14 +46,22 @@ def rel(a, b):
     return float(np.abs(a - b).max() / max(np.abs(a).max(), np.abs(b).max(), 1e-300))
 
 
+def same_J(J, J_h, closure):
+    # the Corey closures' forward steps are the oracle's bits; the van
+    # Genuchten closures' pow is the device library's on the GPU
+    return J == J_h if closure == "corey" else abs(J - J_h) <= 1e-12 * abs(J_h)
+
+
 @pytest.mark.parametrize("kernel", [2, 1, 0])
-@pytest.mark.parametrize("dims", [(20, 17, 12), (70, 19, 21), (9, 11, 1), (1, 12, 9)])
-def test_adjoint_matches_host_build_and_checkpointing_is_exact(kernel, dims):
-    keys, st, lnK, obs, rng = problem(*dims, 31, 9)
+@pytest.mark.parametrize("dims,closure", [((20, 17, 12), "corey"), ((70, 19, 21), "corey"), ((9, 11, 1), "corey"),
+                                          ((1, 12, 9), "corey"), ((20, 17, 12), "reference"),
+                                          ((9, 11, 1), "reference")])
+def test_adjoint_matches_host_build_and_checkpointing_is_exact(kernel, dims, closure):
+    keys, st, lnK, obs, rng = problem(*dims, 31, 9, closure)
     J_h, gK_h, gm_h = emul.gradient(keys, st, 9, obs)
     d = capaz.Domain(keys, st)
     d.tune(adjoint_kernel=kernel)
-    assert d.objective(9, obs) == J_h
+    assert same_J(d.objective(9, obs), J_h, closure)
     grads = []
     # two streams (the segment recompute beside the passes) and one
     for streams in (2, 1):
@@ -63,7 +72,7 @@ def test_adjoint_matches_host_build_and_checkpointing_is_exact(kernel, dims):
             J, gK, gm = d.adjoint(9, obs, checkpoint_every=every)
             # 0: the whole trajectory (a small grid fits), < 0: round(sqrt(9))
             assert d.tuning()["adjoint_segment"] == segment
-            assert J == J_h
+            assert same_J(J, J_h, closure)
             assert rel(gK, gK_h) < 1e-10 and rel(gm, gm_h) < 1e-10, f"checkpoint_every={every} streams={streams}"
             grads.append((gK, gm))
     for gK, gm in grads[1:]:
@@ -77,8 +86,9 @@ def test_adjoint_matches_host_build_and_checkpointing_is_exact(kernel, dims):
     ref.close()
 
 
-def test_adjoint_directional_derivatives_match_central_differences():
-    keys, st, lnK, obs, rng = problem(24, 20, 16, 41, 12)
+@pytest.mark.parametrize("closure", ["corey", "reference"])
+def test_adjoint_directional_derivatives_match_central_differences(closure):
+    keys, st, lnK, obs, rng = problem(24, 20, 16, 41, 12, closure)
     d = capaz.Domain(keys, st)
     J, gK, gm = d.adjoint(12, obs)
 
@@ -94,23 +104,33 @@ def test_adjoint_directional_derivatives_match_central_differences():
     # on this case, checked on the host build) until rounding takes over, so
     # the bar at eps 5e-5 is 1e-4 -- a dropped or mis-signed term is O(1e-2)
     # or worse.
+    # The van Genuchten k_r switch on at an effective saturation of 1e-3, a
+    # jump a central difference can straddle: there the bar holds for the best
+    # of three step sizes (on this case, checked on the host build, one
+    # porosity direction's difference wanders by 1e-4 between steps while the
+    # others agree to 1e-6).
+    def close(fds, ad):
+        return min(abs(fd - ad) for fd in fds) <= 1e-4 * abs(ad)
+
+    steps_K = (5e-5,) if closure == "corey" else (4e-4, 2e-4, 5e-5)
+    steps_m = (1e-4,) if closure == "corey" else (4e-4, 2e-4, 1e-4)
     for _ in range(4):
-        eps = 5e-5
         v = rng.standard_normal(lnK.shape)
-        fd = (J_at(lnK + eps * v, st["m"]) - J_at(lnK - eps * v, st["m"])) / (2 * eps)
+        fds = [(J_at(lnK + e * v, st["m"]) - J_at(lnK - e * v, st["m"])) / (2 * e) for e in steps_K]
         ad = float(np.sum(gK * v))
-        assert abs(fd - ad) <= 1e-4 * abs(fd), (fd, ad)
-        eps = 1e-4
+        assert close(fds, ad), (fds, ad)
         vm = 0.05 * rng.standard_normal(lnK.shape)
-        fd = (J_at(lnK, st["m"] + eps * vm) - J_at(lnK, st["m"] - eps * vm)) / (2 * eps)
+        fds = [(J_at(lnK, st["m"] + e * vm) - J_at(lnK, st["m"] - e * vm)) / (2 * e) for e in steps_m]
         ad = float(np.sum(gm * vm))
-        assert abs(fd - ad) <= 1e-4 * abs(fd), (fd, ad)
+        assert close(fds, ad), (fds, ad)
     d.close()
 
 
-@pytest.mark.parametrize("dims", [(20, 17, 12), (9, 11, 1), (1, 12, 9)])
-def test_tangent_matches_host_build_and_the_adjoint(dims):
-    keys, st, lnK, obs, rng = problem(*dims, 61, 8)
+@pytest.mark.parametrize("dims,closure", [((20, 17, 12), "corey"), ((9, 11, 1), "corey"), ((1, 12, 9), "corey"),
+                                          ((20, 17, 12), "reference")])
+def test_tangent_matches_host_build_and_the_adjoint(dims, closure):
+    keys, st, lnK, obs, rng = problem(*dims, 61, 8, closure)
+    tol_host = 1e-12 if closure == "corey" else 1e-10
     d = capaz.Domain(keys, st)
     J, gK, gm = d.adjoint(8, obs)
     for _ in range(3):
@@ -120,8 +140,8 @@ def test_tangent_matches_host_build_and_the_adjoint(dims):
             d.set_field(f, v)
         Jt, dJ = d.tangent(8, obs, vK, vm)
         J_h, dJ_h = emul.tangent(keys, st, 8, obs, vK, vm)
-        assert Jt == J == J_h
-        assert abs(dJ - dJ_h) <= 1e-12 * abs(dJ_h), (dJ, dJ_h)
+        assert Jt == J and same_J(J, J_h, closure)
+        assert abs(dJ - dJ_h) <= tol_host * abs(dJ_h), (dJ, dJ_h)
         ad = float(np.sum(gK * vK) + np.sum(gm * vm))
         assert abs(dJ - ad) <= 1e-12 * max(abs(dJ), float(np.sum(np.abs(gK * vK)) + np.sum(np.abs(gm * vm)))), \
             (dJ, ad)
@@ -131,6 +151,82 @@ def test_tangent_matches_host_build_and_the_adjoint(dims):
         assert bit_equal(d.field(f), ref.field(f)), f
     d.close()
     ref.close()
+
+
+def problem2(nx, ny, nz, seed, nsteps, **over):
+    keys = case("2ph", nx, ny, nz, CLOSURE="corey", COREY_N=2, S_NR_0=0.0, SOURCE=1, DT=1e-4, ITERATIONS=4)
+    keys.update(over)
+    st = random_state(keys, seed)
+    rng = np.random.default_rng(seed + 1)
+    shape = st["P_w"].shape
+    lnK = np.log(keys["K_0"]) + 0.5 * rng.standard_normal(shape)
+    st["K"] = np.exp(lnK)
+    cells = rng.choice(int(np.prod(shape)), 20, replace=False)
+    steps = sorted({max(1, nsteps // 3), max(1, 2 * nsteps // 3), nsteps})
+    obs = {"cells": cells, "steps": steps, "S_w": rng.uniform(0.6, 0.9, (len(steps), 20)),
+           "S_n": rng.uniform(0.1, 0.4, (len(steps), 20))}
+    return keys, st, lnK, obs, rng
+
+
+@pytest.mark.parametrize("dims,over", [((20, 17, 12), {}), ((70, 19, 21), dict(P_BOTTOM=1.0e5)), ((9, 11, 1), {}),
+                                       ((12, 9, 8), dict(COREY_N=2.6, SOURCE=0))])
+def test_two_phase_adjoint_matches_host_build_and_checkpointing_is_exact(dims, over):
+    """Two phases (Corey closure; S_w = 1 - S_n in the objective): J equals
+    the host build's bit for bit, the gradient matches the host build's to
+    rounding, and neither depends on the segment length or on the second
+    stream."""
+    keys, st, lnK, obs, rng = problem2(*dims, 31, 9, **over)
+    J_h, gK_h, gm_h = emul.gradient(keys, st, 9, obs)
+    d = capaz.Domain(keys, st)
+    assert d.objective(9, obs) == J_h
+    grads = []
+    for streams in (2, 1):
+        d.tune(adjoint_streams=streams)
+        for every, segment in ((0, 9), (-1, 3), (2, 2), (9, 9)):
+            for f, v in st.items():
+                d.set_field(f, v)
+            J, gK, gm = d.adjoint(9, obs, checkpoint_every=every)
+            assert d.tuning()["adjoint_segment"] == segment
+            assert J == J_h
+            assert rel(gK, gK_h) < 1e-10 and rel(gm, gm_h) < 1e-10, f"checkpoint_every={every} streams={streams}"
+            grads.append((gK, gm))
+    for gK, gm in grads[1:]:
+        assert bit_equal(gK, grads[0][0]) and bit_equal(gm, grads[0][1])
+    ref = capaz.Domain(keys, st)
+    ref.run(9)
+    for f in ("P_w", "S_n"):
+        assert bit_equal(d.field(f), ref.field(f)), f
+    d.close()
+    ref.close()
+
+
+def test_two_phase_adjoint_directional_derivatives_match_central_differences():
+    # 4 steps: over 12 the perturbed runs flip upwind choices at faces whose
+    # driving force is near zero, and the central difference straddles those
+    # kinks (on the host build: 5e-4 relative at eps 4e-4 after 12 steps,
+    # <= 1.1e-5 after 4; one step's transposed Jacobian agrees to 1e-9)
+    keys, st, lnK, obs, rng = problem2(24, 20, 16, 41, 4)
+    d = capaz.Domain(keys, st)
+    J, gK, gm = d.adjoint(4, obs)
+
+    def J_at(lnK_, m_):
+        for f, v in st.items():
+            d.set_field(f, v)
+        d.set_field("K", np.exp(lnK_))
+        d.set_field("m", m_)
+        return d.objective(4, obs)
+
+    eps = 4e-4
+    for _ in range(3):
+        v = rng.standard_normal(lnK.shape)
+        fd = (J_at(lnK + eps * v, st["m"]) - J_at(lnK - eps * v, st["m"])) / (2 * eps)
+        ad = float(np.sum(gK * v))
+        assert abs(fd - ad) <= 1e-4 * abs(fd), (fd, ad)
+        vm = 0.05 * rng.standard_normal(lnK.shape)
+        fd = (J_at(lnK, st["m"] + eps * vm) - J_at(lnK, st["m"] - eps * vm)) / (2 * eps)
+        ad = float(np.sum(gm * vm))
+        assert abs(fd - ad) <= 1e-4 * abs(fd), (fd, ad)
+    d.close()
 
 
 def test_history_matching_workload_small():
@@ -150,13 +246,16 @@ def test_history_matching_workload_small():
     d.close()
 
 
-@pytest.mark.parametrize("over,msg", [(dict(MODEL=2), "three-phase"), (dict(CLOSURE="reference"), "Corey"),
+@pytest.mark.parametrize("over,msg", [(dict(MODEL=2, CLOSURE="reference"), "Corey"),
+                                      (dict(ENERGY=1, CLOSURE="reference"), "energy"),
                                       (dict(ITERATIONS=9), "8 Newton")])
 def test_adjoint_rejects_unsupported(over, msg):
     keys, st, lnK, obs, rng = problem(8, 8, 8, 51, 3)
     keys.update(over)
     if over.get("MODEL") == 2:
         st = {k: v for k, v in st.items() if k in ("P_w", "S_n", "m", "K")}
+    if over.get("ENERGY"):
+        st = dict(st, T=np.full(st["P_w"].shape, 285.0))
     d = capaz.Domain(keys, st)
     with pytest.raises(capaz.CapazError, match=msg):
         d.adjoint(3, obs)
