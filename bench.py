@@ -456,11 +456,15 @@ def bench_shard(n: int = 256, gpus: int = 8, steps: int = 200, warmup: int = 20)
     """One GPU's share of the headline workload under a z-slab decomposition
     over `gpus` GPUs, at its real shape: n x n x (n/gpus + 2) cells (the owned
     planes and one ghost plane per neighbour) of the same physics and state,
-    `steps` fused steps timed with HIP events on the context's stream.  After
-    the timed steps, `steps` rounds of the halo's device-side pack and unpack
-    (capaz_load_faces / capaz_save_faces: the two z exchange planes of P_w,
-    S_w, S_n, P_n, P_g, one launch each) timed with HIP events; the
-    GPU-to-GPU transfer itself needs a second GPU and is not run."""
+    `steps` fused steps timed with HIP events on the context's stream.  The
+    halo's device-side work per exchange (the two z exchange planes of P_w,
+    S_w, S_n, P_n, P_g; the GPU-to-GPU transfer itself needs a second GPU and
+    is not run) is what it adds to a step on the GPU's timeline, two ways:
+    pack and unpack as separate launches after the step (capaz_load_faces,
+    capaz_save_faces); the pack written by the step itself
+    (capaz_set_halo_pack) with the unpack (capaz_save_faces) after it; and
+    that pack alone, the transfer landing in the ghost planes
+    (capaz_ghost_planes: a z plane is contiguous, so there is no unpack)."""
     import ctypes
     from capaz_amd import capaz, hip, workloads
 
@@ -481,19 +485,54 @@ def bench_shard(n: int = 256, gpus: int = 8, steps: int = 200, warmup: int = 20)
     ms = ev0.elapsed_ms(ev1) / steps
     L = capaz.load()
     plane = n * n
-    # one exchange: the two planes of P_w, S_w, S_n (and the boundary-persistent
-    # P_n, P_g) packed in one launch, unpacked in one launch
     names = ("P_w", "S_w", "S_n", "P_n", "P_g")
     ids = (ctypes.c_int * len(names))(*[capaz.FIELD_ID[f] for f in names])
     buf = hip.DeviceBuffer(8 * plane * 2 * len(names))
     b = ctypes.c_void_p(buf.address)
-    ev0.record(stream)
-    for _ in range(steps):
+
+    def timed(fn, count):
+        ev0.record(stream)
+        for _ in range(count):
+            fn()
+        ev1.record(stream)
+        d.sync()
+        return ev0.elapsed_ms(ev1) * 1e3 / count
+
+    # per step, on the GPU's timeline (each step's launches take longer than
+    # the host needs to queue an iteration): the step alone; the step, then
+    # the pack and the unpack as separate launches; the step writing the pack
+    # itself, then the unpack; the step writing the pack, with no unpack (the
+    # transfer lands in the ghost planes, capaz_ghost_planes).  Blocks of the
+    # four alternate; the best block of each counts.
+    def step_only(t):
+        d.run(1, first_step=t)
+
+    def separate(t):
+        d.run(1, first_step=t)
         assert L.capaz_load_faces(d._ctx, ids, len(names), 2, b) == 0
         assert L.capaz_save_faces(d._ctx, ids, len(names), 2, b) == 0
-    ev1.record(stream)
-    d.sync()
-    halo_us = ev0.elapsed_ms(ev1) * 1e3 / steps
+
+    def in_step(t):
+        d.run(1, first_step=t)
+        assert L.capaz_save_faces(d._ctx, ids, len(names), 2, b) == 0
+
+    first = warmup + steps + 1
+    block = max(steps // 5, 1)
+    best = {}
+    for _ in range(5):
+        for name, fn in (("step", step_only), ("separate", separate), ("in_step", in_step),
+                         ("in_place", step_only)):
+            d.set_halo_pack(buf.address if name in ("in_step", "in_place") else 0)
+            t0 = first
+
+            def iteration():
+                nonlocal t0
+                fn(t0)
+                t0 += 1
+
+            best[name] = min(best.get(name, float("inf")), timed(iteration, block))
+            first = t0
+    d.set_halo_pack(0)
     buf.close()
     d.close()
     owned = n * n * (nz - 2)
@@ -501,7 +540,9 @@ def bench_shard(n: int = 256, gpus: int = 8, steps: int = 200, warmup: int = 20)
                         f"({nz - 2} owned + 2 ghost planes)",
             "ms_per_step": round(ms, 4), "value": round(owned / (ms / 1e3) / 1e6, 1),
             "unit": "Mcell-steps/s (owned cells)",
-            "halo_pack_unpack_us_per_step": round(halo_us, 2),
+            "halo_pack_unpack_us_per_step": round(best["separate"] - best["step"], 2),
+            "halo_with_in_step_pack_us_per_step": round(best["in_step"] - best["step"], 2),
+            "halo_in_step_pack_no_unpack_us_per_step": round(best["in_place"] - best["step"], 2),
             "halo_bytes_per_step": 2 * len(names) * 8 * plane}
 
 

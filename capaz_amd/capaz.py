@@ -63,6 +63,9 @@ EXPORTS = {
                                         ctypes.c_void_p]),
     "capaz_save_faces": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
                                         ctypes.c_void_p]),
+    "capaz_set_halo_pack": (ctypes.c_int, ["ctx", ctypes.c_void_p]),
+    "capaz_ghost_planes": (ctypes.c_int, ["ctx", ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_int)]),
     "capaz_save_state": (ctypes.c_int, ["ctx", ctypes.c_char_p, ctypes.c_long]),
     "capaz_write_plot_rows": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), ctypes.c_long,
                                              ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
@@ -250,6 +253,19 @@ class Domain:
         s = ctypes.c_void_p()
         self._check(load().capaz_stream(self._ctx, ctypes.byref(s)), "capaz_stream")
         return s.value or 0
+
+    def set_halo_pack(self, address: int) -> None:
+        """Device buffer (address; 0 turns it off) that every step fills with
+        the z exchange planes of its new state, capaz_load_faces' layout."""
+        self._check(load().capaz_set_halo_pack(self._ctx, ctypes.c_void_p(address or None)),
+                    "capaz_set_halo_pack")
+
+    def ghost_planes(self) -> list[int]:
+        """Device addresses of the z ghost planes of the exchange fields in the
+        arrays the next step reads ([field][side], capaz_ghost_planes)."""
+        out, n = (ctypes.c_void_p * 10)(), ctypes.c_int()
+        self._check(load().capaz_ghost_planes(self._ctx, out, 10, ctypes.byref(n)), "capaz_ghost_planes")
+        return [out[q] or 0 for q in range(n.value)]
 
     # -- entry points (defines.h:186-193, 220-222) ----------------------------
     def S_calculation(self):

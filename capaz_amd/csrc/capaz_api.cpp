@@ -19,6 +19,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <new>
 #include <string>
@@ -443,6 +444,9 @@ int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
     if (nsteps < 0) return fail(c, CAPAZ_ERR_INVALID, "nsteps must be >= 0");
     if ((rc = scan_m(c))) return rc;
     using namespace capaz;
+    if (c->halo_pack && nsteps > 0 && (c->user.energy || c->m_has_zero))
+        return fail(c, CAPAZ_ERR_INVALID,
+                    "halo pack: the fused step only (no energy equation, no interior cell with m == 0)");
     if (c->user.energy && c->user.model == 3 && !c->m_has_zero && c->tuning.energy_kernel >= 1) {
         // the energy model's six passes, in place like the sequence
         if ((rc = materialize(c))) return rc;
@@ -494,6 +498,7 @@ int capaz_step(capaz_ctx* c, long first_step, long nsteps) {
         io.Sn1 = c->alt[F_S_n];
         io.Pn1 = c->alt[F_P_n];
         io.Pg1 = c->alt[F_P_g];
+        io.pack = c->halo_pack;
         const KernelEvents* ev;
         if ((rc = next_events(c, &ev))) return rc;
         CAPAZ_HIP(c, launch_fused(c->p, io, c->tuning, c->stream, ev));
@@ -584,6 +589,17 @@ int capaz_stream(capaz_ctx* c, void** s) {
     return CAPAZ_OK;
 }
 
+// Where the interior cells of a state field's current array are stale after
+// fused steps: P_n and P_g of the three-phase model, whose interior values
+// after a step are the previous P_w (P_Xi_ro, three-phase.cpp:230-231, before
+// the Newton) -- the second P_w array, after the step's swap -- while the
+// fused step stores only their boundary values.  NULL where the array is
+// current.
+static const double* stale_inner(capaz_ctx* c, int field) {
+    if (!c->stale || c->user.model != 3 || !c->primary[field]) return nullptr;
+    return field == capaz::F_P_n || field == capaz::F_P_g ? c->alt[capaz::F_P_w] : nullptr;
+}
+
 static int face_call(capaz_ctx* c, int field, int axis, int side, double* buf, int save) {
     int rc;
     if ((rc = check_ctx(c))) return rc;
@@ -598,7 +614,8 @@ static int face_call(capaz_ctx* c, int field, int axis, int side, double* buf, i
     if (!c->primary[field] && (rc = materialize(c))) return rc;
     // load_*: second / second-to-last plane; save_*: first / last plane
     int pos = save ? (side == 0 ? 0 : n - 1) : (side == 0 ? 1 : n - 2);
-    CAPAZ_HIP(c, capaz::launch_face(c->p, c->cur[field], buf, axis, pos, save, c->stream));
+    CAPAZ_HIP(c, capaz::launch_face(c->p, c->cur[field], save ? nullptr : stale_inner(c, field), buf, axis, pos,
+                                    save, c->stream));
     if (save && field == capaz::F_m) c->m_checked = false;
     return CAPAZ_OK;
 }
@@ -626,11 +643,45 @@ static int faces_call(capaz_ctx* c, const int* fields, int nfields, int axis, do
         all_primary = all_primary && c->primary[fields[q]];
     }
     if (!all_primary && (rc = materialize(c))) return rc;
+    const double* inner[capaz::kMaxFaceFields];
     for (int q = 0; q < nfields; ++q) {
         ptr[q] = c->cur[fields[q]];
+        inner[q] = save ? nullptr : stale_inner(c, fields[q]);
         if (save && fields[q] == capaz::F_m) c->m_checked = false;
     }
-    CAPAZ_HIP(c, capaz::launch_faces(c->p, ptr, nfields, buf, axis, save, c->stream));
+    CAPAZ_HIP(c, capaz::launch_faces(c->p, ptr, inner, nfields, buf, axis, save, c->stream));
+    return CAPAZ_OK;
+}
+
+int capaz_set_halo_pack(capaz_ctx* c, double* dbuf) {
+    int rc;
+    if ((rc = check_ctx(c))) return rc;
+    if (dbuf) {
+        if (c->user.energy) return fail(c, CAPAZ_ERR_INVALID, "halo pack: models without the energy equation only");
+        if (c->p.Nz < 3) return fail(c, CAPAZ_ERR_INVALID, "halo pack: Nz must be at least 3");
+        if (reinterpret_cast<uintptr_t>(dbuf) % alignof(double))
+            return fail(c, CAPAZ_ERR_INVALID, "halo pack: buffer not aligned to 8 bytes");
+    }
+    c->halo_pack = dbuf;
+    return CAPAZ_OK;
+}
+
+int capaz_ghost_planes(capaz_ctx* c, double** dst, int nmax, int* n) {
+    int rc;
+    if ((rc = check_ctx(c))) return rc;
+    if (!dst || !n) return fail(c, CAPAZ_ERR_INVALID, "ghost planes: NULL argument");
+    if (c->user.energy) return fail(c, CAPAZ_ERR_INVALID, "ghost planes: models without the energy equation only");
+    if (c->p.Nz < 3) return fail(c, CAPAZ_ERR_INVALID, "ghost planes: Nz must be at least 3");
+    using namespace capaz;
+    static const int three[] = {F_P_w, F_S_w, F_S_n, F_P_n, F_P_g}, two[] = {F_P_w, F_S_n};
+    const int* f = c->user.model == 3 ? three : two;
+    const int nf = c->user.model == 3 ? 5 : 2;
+    if (nmax < 2 * nf) return fail(c, CAPAZ_ERR_INVALID, "ghost planes: dst has room for fewer than 2 x fields");
+    for (int q = 0; q < nf; ++q) {
+        dst[2 * q] = c->cur[f[q]];
+        dst[2 * q + 1] = c->cur[f[q]] + (int64_t)(c->p.Nz - 1) * c->p.sz;
+    }
+    *n = 2 * nf;
     return CAPAZ_OK;
 }
 

@@ -453,6 +453,9 @@ struct StepIO {
     // Xi and ro per phase as stored by P_Xi_ro (read by the fused z kernel's
     // stored-coefficient variant only, the energy step's roS pass)
     const double *Xi[3] = {}, *ro[3] = {};
+    // non-NULL: the step also writes the z exchange planes of its new state
+    // here (fused_pack)
+    double* pack = nullptr;
 };
 
 // assign_P_Xi + assign_ro of one cell from the current primary state.

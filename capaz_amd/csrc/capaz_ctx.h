@@ -42,6 +42,9 @@ struct capaz_ctx {
     // [2..3] segment buffer b free again
     hipStream_t adj_stream = nullptr;
     hipEvent_t adj_ev[4] = {};
+    // caller's device buffer for the z exchange planes each step writes
+    // (capaz_set_halo_pack), or NULL
+    double* halo_pack = nullptr;
     std::string err;
 };
 

@@ -137,15 +137,14 @@ __device__ __forceinline__ bool inner_k(const DevParams& p, int k) {
     return ((k != 0) && (k != p.Nz - 1)) || (p.Nz < 2);
 }
 
-// Boundary cell (bi, bj, bk) from its inward neighbour's new state.
+// The new state (P_w, S_w, S_n, P_n, P_g) of boundary cell (bi, bj, bk) from
+// its inward neighbour's new state (two phases: v[0] = P_w, v[2] = S_n).
 template <int MODEL>
-__device__ __forceinline__ void write_boundary(const DevParams& p, const StepIO& io, int bi, int bj, int bk,
-                                               double Pw, double Sw, double Sn, double ro_eff) {
-    const int64_t lb = (int64_t)bi + (int64_t)bj * p.sy + (int64_t)bk * p.sz;
+__device__ __forceinline__ void boundary_values(const DevParams& p, int bi, int bj, int bk, double Pw, double Sw,
+                                                double Sn, double ro_eff, double* v) {
     if constexpr (MODEL == 2) {
         double Snb = Sn;
         if (bj == 0 && p.source > 0 && in_source_window(p, bi, bk)) Snb = p.S_n_gr;
-        io.Sn1[lb] = Snb;
         double Pb;
         if ((bj != 0) && (bj != p.Ny - 1))
             Pb = Pw;
@@ -155,20 +154,35 @@ __device__ __forceinline__ void write_boundary(const DevParams& p, const StepIO&
             Pb = p.P_bottom;
         else
             Pb = Pw + ro_eff;
-        io.Pw1[lb] = Pb;
+        v[0] = Pb;
+        v[2] = Snb;
     } else {
+        (void)bi;
+        (void)bk;
         if (bj == 0 && p.source > 0) {
-            io.Sw1[lb] = p.S_w_gr;
-            io.Sn1[lb] = p.S_n_gr;
+            v[1] = p.S_w_gr;
+            v[2] = p.S_n_gr;
         } else {
-            io.Sw1[lb] = Sw;
-            io.Sn1[lb] = Sn;
+            v[1] = Sw;
+            v[2] = Sn;
         }
-        double Pwb, Pnb, Pgb;
-        border_p3(p, bj, Pw, Pwb, Pnb, Pgb);
-        io.Pw1[lb] = Pwb;
-        io.Pn1[lb] = Pnb;
-        io.Pg1[lb] = Pgb;
+        border_p3(p, bj, Pw, v[0], v[3], v[4]);
+    }
+}
+
+// Boundary cell (bi, bj, bk) from its inward neighbour's new state.
+template <int MODEL>
+__device__ __forceinline__ void write_boundary(const DevParams& p, const StepIO& io, int bi, int bj, int bk,
+                                               double Pw, double Sw, double Sn, double ro_eff) {
+    const int64_t lb = (int64_t)bi + (int64_t)bj * p.sy + (int64_t)bk * p.sz;
+    double v[5];
+    boundary_values<MODEL>(p, bi, bj, bk, Pw, Sw, Sn, ro_eff, v);
+    io.Pw1[lb] = v[0];
+    io.Sn1[lb] = v[2];
+    if constexpr (MODEL == 3) {
+        io.Sw1[lb] = v[1];
+        io.Pn1[lb] = v[3];
+        io.Pg1[lb] = v[4];
     }
 }
 
@@ -690,6 +704,65 @@ inline int64_t fused_face_cells(const DevParams& p) {
     const bool hasx = p.Nx >= 3, hasz = p.Nz >= 3;
     const int64_t ni = hasx ? p.Nx - 2 : p.Nx;
     return 2LL * p.Nx * p.Nz + (hasx ? 2LL * (p.Ny - 2) * p.Nz : 0) + (hasz ? 2LL * ni * (p.Ny - 2) : 0);
+}
+
+// The halo pack of a z-slab exchange written by the step itself: cell c of
+// side s (plane 1, or plane Nz - 2) of the new state into io.pack at
+// (f * 2 + s) * Nx * Ny + c for the state fields f (three phases: P_w, S_w,
+// S_n, P_n, P_g; two: P_w, S_n) -- capaz_load_faces' layout for axis 2.
+// Interior cells give their new values (P_n = P_g = P_w, as the step takes
+// them), the plane's x / y boundary cells the Border values of their inward
+// cell, computed here rather than read, so the pack needs no second launch
+// after the boundary writes.
+template <int MODEL, int CLOSURE>
+__device__ __forceinline__ void fused_pack(const DevParams& p, const StepIO& io, int64_t t) {
+    const int64_t plane = (int64_t)p.Nx * p.Ny;
+    if (t >= 2 * plane) return;
+    const int s = t >= plane ? 1 : 0;
+    const int64_t c = t - s * plane;
+    const int k = s ? p.Nz - 2 : 1;
+    const int j = (int)(c / p.Nx), i = (int)(c - (int64_t)j * p.Nx);
+    const bool interior = (p.Nx < 3 || (i != 0 && i != p.Nx - 1)) && j != 0 && j != p.Ny - 1;
+    const int64_t l = (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz;
+    double v[5];
+    if (interior) {
+        v[0] = io.Pw1[l];
+        v[2] = io.Sn1[l];
+        if constexpr (MODEL == 3) {
+            v[1] = io.Sw1[l];
+            v[3] = v[4] = v[0];
+        }
+    } else {
+        const int64_t l1 = inward(p, i, j, k);
+        const double Pw = io.Pw1[l1], Sn = io.Sn1[l1];
+        const double Sw = MODEL == 3 ? io.Sw1[l1] : 0.;
+        double ro_eff = 0.;
+        if constexpr (MODEL == 2) {
+            if (j == p.Ny - 1) {
+                const CoefT<2> c1 = coef_at<2, CLOSURE>(p, io, l1, true);
+                ro_eff = (c1.ro[1] * Sn + c1.ro[0] * (1 - Sn)) * io.m[l1] * p.g * p.hy;
+            }
+        }
+        boundary_values<MODEL>(p, i, j, k, Pw, Sw, Sn, ro_eff, v);
+    }
+    if constexpr (MODEL == 3) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) io.pack[(f * 2 + s) * plane + c] = v[f];
+    } else {
+        io.pack[s * plane + c] = v[0];
+        io.pack[(2 + s) * plane + c] = v[2];
+    }
+}
+
+// k_fused_faces followed, in the same launch, by the halo pack (threads past
+// the face cells).
+template <int MODEL, int CLOSURE>
+__global__ void __launch_bounds__(256) k_fused_faces_pack(DevParams p, StepIO io, int64_t nfaces) {
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (t < nfaces)
+        fused_face<MODEL, CLOSURE>(p, io, t);
+    else
+        fused_pack<MODEL, CLOSURE>(p, io, t - nfaces);
 }
 
 }  // namespace capaz

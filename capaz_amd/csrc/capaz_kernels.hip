@@ -122,11 +122,28 @@ __global__ void __launch_bounds__(kBlock) k_m_zero(DevParams p, const double* m,
     if (cell_inner(p, i, j, k) && !m[l]) atomicOr(flag, 1);
 }
 
+// field[l], or inner[l] where inner is set and the cell is interior.  Both
+// loads are made under a uniform test of inner and the choice is a per-lane
+// select of values: a conditional choice between the two base pointers was
+// compiled into a uniform pointer (every lane reading inner).
+__device__ __forceinline__ double pick_inner(const double* field, const double* inner, int64_t l, bool interior) {
+    double v = field[l];
+    if (inner) {
+        const double w = inner[l];
+        v = interior ? w : v;
+    }
+    return v;
+}
+
 // Face planes (cpu.cpp:769-888): plane index `pos` normal to `axis`; the
 // buffer is indexed (a + na*b) with (a, b) = (j, k) for x, (i, k) for y,
-// (i, j) for z.
-__global__ void __launch_bounds__(kBlock) k_face(DevParams p, double* field, double* buf, int axis, int pos,
-                                                 int save) {
+// (i, j) for z.  A load with `inner` set takes the plane's interior cells
+// (interior3) from `inner` instead of the field: P_n and P_g of a
+// three-phase state after fused steps, whose interior values are the
+// previous P_w (what the last step's P_Xi_ro gave them; the fused step
+// keeps them in registers and stores only their boundary values).
+__global__ void __launch_bounds__(kBlock) k_face(DevParams p, double* field, const double* inner, double* buf,
+                                                 int axis, int pos, int save) {
     int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     int na = axis == 0 ? p.Ny : p.Nx;
     int nb = axis == 2 ? p.Ny : p.Nz;
@@ -137,10 +154,11 @@ __global__ void __launch_bounds__(kBlock) k_face(DevParams p, double* field, dou
     else if (axis == 1) { i = a; j = pos; k = b; }
     else { i = a; j = b; k = pos; }
     int64_t l = (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz;
-    if (save)
+    if (save) {
         field[l] = buf[t];
-    else
-        buf[t] = field[l];
+        return;
+    }
+    buf[t] = pick_inner(field, inner, l, interior3(p, i, j, k));
 }
 
 // All the exchange planes of one halo exchange in one launch.  Grid row y
@@ -149,9 +167,10 @@ __global__ void __launch_bounds__(kBlock) k_face(DevParams p, double* field, dou
 // field pointer and the plane position are uniform over a workgroup.  A z
 // plane is contiguous in the field too (sy == Nx); with vec set (checked on
 // the host: even plane, 16-byte aligned field and buffer) each lane moves two
-// doubles of it.
+// doubles of it.  inner[q]: as k_face's `inner`, for field q of a load.
 struct FaceFields {
     double* f[kMaxFaceFields];
+    const double* inner[kMaxFaceFields];
 };
 __global__ void __launch_bounds__(kBlock) k_faces(DevParams p, FaceFields ff, double* buf, int axis, int save,
                                                   int vec) {
@@ -162,6 +181,7 @@ __global__ void __launch_bounds__(kBlock) k_faces(DevParams p, FaceFields ff, do
     // load: second / second-to-last plane; save: first / last plane
     const int pos = save ? (s == 0 ? 0 : n - 1) : (s == 0 ? 1 : n - 2);
     double* const f = ff.f[q];
+    const double* const g = save ? nullptr : ff.inner[q];
     double* const side = buf + (int64_t)y * plane;
     const uint32_t t = blockIdx.x * (uint32_t)kBlock + threadIdx.x;
     if (axis == 2 && p.sy == p.Nx) {
@@ -171,25 +191,32 @@ __global__ void __launch_bounds__(kBlock) k_faces(DevParams p, FaceFields ff, do
             if (c >= plane) return;
             double2* dst = reinterpret_cast<double2*>(save ? fp + c : side + c);
             const double2* src = reinterpret_cast<const double2*>(save ? side + c : fp + c);
-            *dst = *src;
+            double2 v = *src;
+            if (g) {
+                const double2 w = *reinterpret_cast<const double2*>(g + (int64_t)pos * p.sz + c);
+                v.x = interior3(p, (int)(c % na), (int)(c / na), pos) ? w.x : v.x;
+                v.y = interior3(p, (int)((c + 1) % na), (int)((c + 1) / na), pos) ? w.y : v.y;
+            }
+            *dst = v;
         } else if (t < plane) {
             if (save)
                 fp[t] = side[t];
             else
-                side[t] = fp[t];
+                side[t] = pick_inner(f, g, (int64_t)pos * p.sz + t, interior3(p, (int)(t % na), (int)(t / na), pos));
         }
         return;
     }
     if (t >= plane) return;
     const uint32_t a = t % na, b = t / na;
-    int64_t l;
-    if (axis == 0) l = (int64_t)pos + (int64_t)a * p.sy + (int64_t)b * p.sz;
-    else if (axis == 1) l = (int64_t)a + (int64_t)pos * p.sy + (int64_t)b * p.sz;
-    else l = (int64_t)a + (int64_t)b * p.sy + (int64_t)pos * p.sz;
+    int i, j, k;
+    if (axis == 0) { i = pos; j = (int)a; k = (int)b; }
+    else if (axis == 1) { i = (int)a; j = pos; k = (int)b; }
+    else { i = (int)a; j = (int)b; k = pos; }
+    const int64_t l = (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz;
     if (save)
         f[l] = side[t];
     else
-        side[t] = f[l];
+        side[t] = pick_inner(f, g, l, interior3(p, i, j, k));
 }
 
 // Non-finite sentinel: the smallest cell index at which one of up to four
@@ -278,6 +305,25 @@ int fused_kchunk(const DevParams& p, int kchunk, int ty, int resident) {
     return best;
 }
 
+// The halo pack after a step whose launches wrote every new cell (nothing
+// without io.pack).
+template <int MODEL, int CLOSURE>
+__global__ void __launch_bounds__(256) k_fused_pack(DevParams p, StepIO io) {
+    fused_pack<MODEL, CLOSURE>(p, io, (int64_t)blockIdx.x * 256 + threadIdx.x);
+}
+template <int MODEL, int CLOSURE>
+hipError_t launch_pack(const DevParams& p, const StepIO& io, hipStream_t s) {
+    if (!io.pack) return hipSuccess;
+    hipLaunchKernelGGL((k_fused_pack<MODEL, CLOSURE>), grid_for(2LL * p.Nx * p.Ny), dim3(256), 0, s, p, io);
+    return hipGetLastError();
+}
+template <int MODEL>
+hipError_t launch_pack_model(const DevParams& p, const StepIO& io, hipStream_t s) {
+    if (p.closure == 0) return launch_pack<MODEL, 0>(p, io, s);
+    if (p.corey_n == 2.) return launch_pack<MODEL, 2>(p, io, s);
+    return launch_pack<MODEL, 1>(p, io, s);
+}
+
 template <int MODEL, int CLOSURE, int TY, bool CARRY>
 hipError_t launch_z(const DevParams& p, const StepIO& io, int kchunk, hipStream_t s, const KernelEvents* ev) {
     const int nx = p.Nx >= 3 ? p.Nx - 2 : 1, ny = p.Ny - 2, nz = p.Nz >= 3 ? p.Nz - 2 : 1;
@@ -289,9 +335,16 @@ hipError_t launch_z(const DevParams& p, const StepIO& io, int kchunk, hipStream_
     hipLaunchKernelGGL(kernel, grid, dim3(64, TY), 0, s, p, io, kchunk);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (ev && (e = hipEventRecord(ev->end, s)) != hipSuccess) return e;
-    if constexpr (!kFusedFaces<MODEL>)
-        hipLaunchKernelGGL((k_fused_faces<MODEL, CLOSURE>), grid_for(fused_face_cells(p)), dim3(256), 0, s, p, io);
-    return hipGetLastError();
+    if constexpr (!kFusedFaces<MODEL>) {
+        const int64_t nfaces = fused_face_cells(p);
+        if (io.pack)
+            hipLaunchKernelGGL((k_fused_faces_pack<MODEL, CLOSURE>), grid_for(nfaces + 2LL * p.Nx * p.Ny), dim3(256),
+                               0, s, p, io, nfaces);
+        else
+            hipLaunchKernelGGL((k_fused_faces<MODEL, CLOSURE>), grid_for(nfaces), dim3(256), 0, s, p, io);
+        return hipGetLastError();
+    }
+    return launch_pack<MODEL, CLOSURE>(p, io, s);
 }
 
 template <int MODEL, int CLOSURE>
@@ -318,7 +371,8 @@ hipError_t launch_fused(const DevParams& p, const StepIO& io, const Tuning& t, h
         hipError_t e;
         if (ev && (e = hipEventRecord(ev->begin, s)) != hipSuccess) return e;
         if ((e = launch_fused_step(p, io, s)) != hipSuccess) return e;
-        return ev ? hipEventRecord(ev->end, s) : hipSuccess;
+        if (ev && (e = hipEventRecord(ev->end, s)) != hipSuccess) return e;
+        return p.model == 3 ? launch_pack_model<3>(p, io, s) : launch_pack_model<2>(p, io, s);
     }
     return p.model == 3 ? launch_z_model<3>(p, io, t, s, ev) : launch_z_model<2>(p, io, t, s, ev);
 }
@@ -472,8 +526,8 @@ hipError_t launch_find_nonfinite(const DevParams& p, const double* const* f, uns
     return hipGetLastError();
 }
 
-hipError_t launch_faces(const DevParams& p, double* const* fields, int nfields, double* buf, int axis, int save,
-                        hipStream_t s) {
+hipError_t launch_faces(const DevParams& p, double* const* fields, const double* const* inner, int nfields,
+                        double* buf, int axis, int save, hipStream_t s) {
     if (nfields < 1 || nfields > kMaxFaceFields) return hipErrorInvalidValue;
     const int64_t na = axis == 0 ? p.Ny : p.Nx, nb = axis == 2 ? p.Ny : p.Nz, plane = na * nb;
     if (plane * 2 * nfields >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
@@ -481,7 +535,8 @@ hipError_t launch_faces(const DevParams& p, double* const* fields, int nfields, 
     bool aligned = ((uintptr_t)buf & 15) == 0;
     for (int q = 0; q < nfields; ++q) {
         ff.f[q] = fields[q];
-        aligned = aligned && ((uintptr_t)fields[q] & 15) == 0;
+        ff.inner[q] = inner ? inner[q] : nullptr;
+        aligned = aligned && ((uintptr_t)fields[q] & 15) == 0 && ((uintptr_t)ff.inner[q] & 15) == 0;
     }
     const int vec = axis == 2 && p.sy == p.Nx && plane % 2 == 0 && p.sz % 2 == 0 && aligned;
     const int64_t lanes = vec ? plane / 2 : plane;
@@ -490,9 +545,10 @@ hipError_t launch_faces(const DevParams& p, double* const* fields, int nfields, 
     return hipGetLastError();
 }
 
-hipError_t launch_face(const DevParams& p, double* field, double* buf, int axis, int pos, int save, hipStream_t s) {
+hipError_t launch_face(const DevParams& p, double* field, const double* inner, double* buf, int axis, int pos,
+                       int save, hipStream_t s) {
     int64_t na = axis == 0 ? p.Ny : p.Nx, nb = axis == 2 ? p.Ny : p.Nz;
-    hipLaunchKernelGGL(k_face, grid_for(na * nb), dim3(kBlock), 0, s, p, field, buf, axis, pos, save);
+    hipLaunchKernelGGL(k_face, grid_for(na * nb), dim3(kBlock), 0, s, p, field, inner, buf, axis, pos, save);
     return hipGetLastError();
 }
 

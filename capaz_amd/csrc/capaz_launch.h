@@ -85,12 +85,16 @@ hipError_t launch_fused_record(const DevParams& p, const StepIO& io, int kchunk,
 hipError_t launch_dtm(const DevParams& p, const double* m, double* dtm, hipStream_t s);
 hipError_t launch_copy_interior(const DevParams& p, const double* src, double* dst, hipStream_t s);
 hipError_t launch_m_zero(const DevParams& p, const double* m, int* flag, hipStream_t s);
-hipError_t launch_face(const DevParams& p, double* field, double* buf, int axis, int pos, int save, hipStream_t s);
+// One face plane; a load with `inner` set reads the plane's interior cells
+// from it instead of the field.
+hipError_t launch_face(const DevParams& p, double* field, const double* inner, double* buf, int axis, int pos,
+                       int save, hipStream_t s);
 // Both exchange planes of nfields (<= kMaxFaceFields) fields along `axis` in
-// one launch, buffer layout [field][side][plane].
+// one launch, buffer layout [field][side][plane]; inner (NULL, or NULL or a
+// source per field) as launch_face's.
 constexpr int kMaxFaceFields = 8;
-hipError_t launch_faces(const DevParams& p, double* const* fields, int nfields, double* buf, int axis, int save,
-                        hipStream_t s);
+hipError_t launch_faces(const DevParams& p, double* const* fields, const double* const* inner, int nfields,
+                        double* buf, int axis, int save, hipStream_t s);
 // *first = the smallest cell index at which f[0], f[1] or (where not NULL)
 // f[2], f[3] is not finite, ~0 if there is none.
 hipError_t launch_find_nonfinite(const DevParams& p, const double* const* f, unsigned long long* first,

@@ -38,6 +38,11 @@ def _ok(rc: int, what: str) -> None:
         raise RuntimeError(f"{what}: {lib().hipGetErrorString(rc).decode()}")
 
 
+def copy(dst: int, src: int, nbytes: int) -> None:
+    """Device-to-device copy (hipMemcpy; returns when it is done)."""
+    _ok(lib().hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), nbytes, D2D), "hipMemcpy")
+
+
 class DeviceBuffer:
     """nbytes of device memory (hipMalloc), freed on close()."""
 

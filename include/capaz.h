@@ -282,6 +282,33 @@ int capaz_save_face(capaz_ctx* ctx, int field, int axis, int side, const double*
 int capaz_load_faces(capaz_ctx* ctx, const int* fields, int nfields, int axis, double* dbuf);
 int capaz_save_faces(capaz_ctx* ctx, const int* fields, int nfields, int axis, const double* dbuf);
 
+/* The z exchange's pack written by the step itself (no counterpart in the
+ * reference, whose exchange loads the planes after the step; for a z-slab
+ * decomposition, one slab per GPU): with dbuf set, every capaz_step also
+ * writes planes 1 and Nz-2 of the new state into dbuf, laid out as
+ * capaz_load_faces(axis 2) of the fields P_w, S_w, S_n, P_n, P_g (three
+ * phases; 10 planes) or P_w, S_n (two phases; 4 planes), in the launch that
+ * writes the boundary faces (three phases) or one small launch after the step
+ * (two phases).  The values are those capaz_load_faces returns after the
+ * step, except P_n and P_g of the planes' interior cells, which hold the new
+ * P_w (the value the next step gives them) instead of the previous one.
+ * dbuf NULL turns it off.  The energy models, and contexts whose m has a zero
+ * in an interior cell, are rejected (CAPAZ_ERR_INVALID, at capaz_step for
+ * the latter); Nz must be at least 3. */
+int capaz_set_halo_pack(capaz_ctx* ctx, double* dbuf);
+
+/* The receiving end of that exchange, with no unpack (no counterpart in the
+ * reference, whose exchange unpacks a host buffer): the device addresses of
+ * the ghost planes -- plane 0 in dst[2q], plane Nz-1 in dst[2q+1] -- of the
+ * exchange fields q of capaz_set_halo_pack's order, in the arrays the next
+ * capaz_step reads.  A z plane of a field is Nx*Ny contiguous doubles, so a
+ * neighbour's pack planes can be transferred (hipMemcpyPeerAsync, an RCCL
+ * receive) straight into them.  The addresses alternate between two arrays
+ * from step to step: ask again after every capaz_step.  nmax: room in dst
+ * (10 for three phases, 4 for two); *n receives the count.  The same models
+ * and Nz as capaz_set_halo_pack. */
+int capaz_ghost_planes(capaz_ctx* ctx, double** dst, int nmax, int* n);
+
 /* ---- Checkpoint / restart ---------------------------------------------------
  * Replace save / restore (main.cpp:1035-1145) with a format that keeps every
  * field the next step reads (the primary state, including the boundary

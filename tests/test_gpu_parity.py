@@ -378,7 +378,8 @@ def test_batched_faces_equal_the_per_face_calls(axis, dims, offset):
     [field][side][plane], as the per-field, per-side capaz_load_face calls
     lay them out; capaz_save_faces writes the first and last planes as
     capaz_save_face does.  After fused steps (ping-pong state, stale
-    intermediates) both see the current state.  z planes of an even size in a
+    intermediates) both see the current state, the planes of the fields as
+    read after the step.  z planes of an even size in a
     16-byte aligned buffer take the two-doubles-per-lane path; an odd plane
     (9 x 7) and a buffer 8 bytes off alignment take the one-double path."""
     import ctypes
@@ -407,6 +408,14 @@ def test_batched_faces_equal_the_per_face_calls(axis, dims, offset):
             assert L.capaz_load_face(d._ctx, ids[q], axis, s, ctypes.c_void_p(one.address)) == 0
             d.sync()
             np.testing.assert_array_equal(got[q, s], one.download(plane))
+    # the loads saw the state as the reference holds it after the step (P_n,
+    # P_g interior: the previous P_w), which reading the fields materializes
+    n_axis = {0: nx, 1: ny, 2: nz}[axis]
+    for q, f in enumerate(names):
+        full = d.field(f)
+        for s, pos in ((0, 1), (1, n_axis - 2)):
+            want = full[:, :, pos] if axis == 0 else (full[:, pos, :] if axis == 1 else full[pos, :, :])
+            np.testing.assert_array_equal(got[q, s], want.reshape(-1), err_msg=f"{f} side {s}")
     # unpack a known buffer and compare with the per-face writes on a twin
     e = capaz.Domain(keys, st)
     e.run(3)
@@ -424,6 +433,115 @@ def test_batched_faces_equal_the_per_face_calls(axis, dims, offset):
     one.close()
     d.close()
     e.close()
+
+
+PACK_CASES = {
+    "3ph-corey2": ("3ph", (9, 8, 7), dict(CLOSURE="corey", S_WR_0=0.1, S_NR_0=0.05), {}),
+    "3ph-corey2-nx1": ("3ph", (1, 8, 7), dict(CLOSURE="corey", S_WR_0=0.1, S_NR_0=0.05), {}),
+    "3ph-corey2.5": ("3ph", (9, 7, 6), dict(CLOSURE="corey", COREY_N=2.5, S_WR_0=0.1, S_NR_0=0.05), {}),
+    "3ph-reference": ("3ph", (9, 8, 7), dict(SOURCE=1), {}),
+    "3ph-corey2-cellkernel": ("3ph", (9, 8, 7), dict(CLOSURE="corey", S_WR_0=0.1, S_NR_0=0.05),
+                              dict(fused_kernel=0)),
+    "3ph-corey2-nocarry": ("3ph", (9, 8, 7), dict(CLOSURE="corey"), dict(fused_kernel=2, tile_y=4)),
+    "2ph-reference": ("2ph", (9, 8, 7), {}, {}),
+    "2ph-corey": ("2ph", (9, 7, 6), dict(CLOSURE="corey", COREY_N=2.0), {}),
+}
+
+
+@pytest.mark.parametrize("name", list(PACK_CASES))
+def test_halo_pack_equals_load_faces_after_each_step(name):
+    """capaz_set_halo_pack: every step writes planes 1 and Nz-2 of its new
+    state into the buffer -- capaz_load_faces(axis 2) of the state fields
+    after that step, bit for bit, except P_n and P_g of the planes' interior
+    cells, which hold the new P_w.  The state itself is unchanged by the pack
+    (a twin without it agrees bit for bit), and each step rewrites it.  The
+    pack copied straight into the ghost planes that capaz_ghost_planes names
+    gives the next step what capaz_save_faces of the same buffer gives it."""
+    import ctypes
+    from capaz_amd import hip
+    model, (nx, ny, nz), over, tune = PACK_CASES[name]
+    keys = case(model, nx, ny, nz, **over)
+    st = random_state(keys, 11)
+    d, e = capaz.Domain(keys, st), capaz.Domain(keys, st)
+    for dom in (d, e):
+        if tune:
+            dom.tune(**tune)
+    names = ("P_w", "S_w", "S_n", "P_n", "P_g") if model == "3ph" else ("P_w", "S_n")
+    plane = nx * ny
+    pack, ref = hip.DeviceBuffer(8 * plane * 2 * len(names)), hip.DeviceBuffer(8 * plane * 2 * len(names))
+    d.set_halo_pack(pack.address)
+    L = capaz.load()
+    ids = (ctypes.c_int * len(names))(*[capaz.FIELD_ID[f] for f in names])
+    jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    inner = ((jj > 0) & (jj < ny - 1) & (((ii > 0) & (ii < nx - 1)) | (nx < 3))).reshape(-1)
+    for step in range(1, 4):
+        d.run(1, first_step=step)
+        e.run(1, first_step=step)
+        assert L.capaz_load_faces(e._ctx, ids, len(names), 2, ctypes.c_void_p(ref.address)) == 0
+        d.sync()
+        e.sync()
+        got = pack.download(plane * 2 * len(names)).reshape(len(names), 2, plane)
+        want = ref.download(plane * 2 * len(names)).reshape(len(names), 2, plane)
+        for q, f in enumerate(names):
+            w = want[q].copy()
+            if f in ("P_n", "P_g"):
+                w[:, inner] = want[0][:, inner]
+            assert bit_equal(got[q], w), f"{name} step {step}: field {f}"
+        for f in names:
+            assert bit_equal(d.field(f), e.field(f)), f"{name} step {step}: state {f}"
+        # the exchange's receiving end: d takes the pack straight into its
+        # ghost planes (capaz_ghost_planes), e through capaz_save_faces; the
+        # next step must see the same ghost values in both
+        ghosts = d.ghost_planes()
+        assert len(ghosts) == 2 * len(names)
+        for q in range(len(names)):
+            for side in (0, 1):
+                hip.copy(ghosts[2 * q + side], pack.address + 8 * plane * (2 * q + side), 8 * plane)
+        ref.upload(got.reshape(-1))
+        assert L.capaz_save_faces(e._ctx, ids, len(names), 2, ctypes.c_void_p(ref.address)) == 0
+        e.sync()
+    d.run(1, first_step=4)
+    e.run(1, first_step=4)
+    for f in names:
+        assert bit_equal(d.field(f), e.field(f)), f"{name} after the exchanges: state {f}"
+    got = pack.download(plane * 2 * len(names)).reshape(len(names), 2, plane)
+    d.set_halo_pack(0)
+    d.run(1, first_step=5)
+    d.sync()
+    assert bit_equal(pack.download(plane * 2 * len(names)).reshape(len(names), 2, plane),
+                     got), "the pack was written after it was turned off"
+    pack.close()
+    ref.close()
+    d.close()
+    e.close()
+
+
+def test_halo_pack_rejections():
+    """The energy models and m == 0 interior cells are rejected (the step
+    takes the entry-point sequence there), as is Nz below 3."""
+    from capaz_amd import hip
+    buf = hip.DeviceBuffer(8 * 1024)
+    d = capaz.Domain(case("3phe", 5, 5, 5), random_state(case("3phe", 5, 5, 5), 1))
+    with pytest.raises(RuntimeError, match="energy"):
+        d.set_halo_pack(buf.address)
+    with pytest.raises(RuntimeError, match="energy"):
+        d.ghost_planes()
+    d.close()
+    d = capaz.Domain(case("3ph", 5, 5, 1), random_state(case("3ph", 5, 5, 1), 1))
+    with pytest.raises(RuntimeError, match="Nz"):
+        d.set_halo_pack(buf.address)
+    d.close()
+    keys = case("3ph", 5, 5, 5)
+    st = random_state(keys, 1)
+    st["m"][2, 2, 2] = 0.0
+    d = capaz.Domain(keys, st)
+    d.set_halo_pack(buf.address)
+    with pytest.raises(RuntimeError, match="m == 0"):
+        d.run(1)
+    d.set_halo_pack(0)
+    d.run(1)
+    d.close()
+    buf.close()
 
 
 def test_invalid_field_reports_error():
