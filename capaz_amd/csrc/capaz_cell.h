@@ -243,12 +243,20 @@ CAPAZ_HD double ro_g_T(const DevParams& p, double P, double T) {
 // ---- Darcy flux (cpu.cpp:236-264, 619-731) -----------------------------------------
 
 // directed_difference: x1, x2 are the left/right driving forces, (Xi, ro) of
-// the cell (c) and of its minus (m) and plus (p) neighbours.
+// the cell (c) and of its minus (m) and plus (p) neighbours.  The
+// reference's upwind parts (x + |x|) / 2 and (x - |x|) / 2 are max(x, 0) and
+// min(x, 0): for |x| < 2^1023, x +- |x| is 2x (exact) or +-0 and the halving
+// is exact, so they agree up to the sign of a zero.  A NaN x (a NaN pressure
+// at c or at the neighbour) gives NaN parts in the reference and 0 here; the
+// flux is NaN either way, through the density of the same cell (ro_c, or
+// ro_m / ro_p times the 0), a function of that pressure.
+CAPAZ_HD double pos_part(double x) { return fmax(x, 0.); }
+CAPAZ_HD double neg_part(double x) { return fmin(x, 0.); }
 CAPAZ_HD double directed(double x1, double x2, double Xic, double roc, double Xim, double rom, double Xip, double rop,
                          double h, double rh) {
-    double t1 = (half(x2 + fabs(x2)) - half(x1 - fabs(x1))) * (-1.) * Xic * roc;
-    double t2 = half(x1 + fabs(x1)) * (-1.) * Xim * rom;
-    double t3 = half(x2 - fabs(x2)) * (-1.) * Xip * rop;
+    double t1 = (pos_part(x2) - neg_part(x1)) * (-1.) * Xic * roc;
+    double t2 = pos_part(x1) * (-1.) * Xim * rom;
+    double t3 = neg_part(x2) * (-1.) * Xip * rop;
     return exact_div(t1 - t2 + t3, h, rh);
 }
 
