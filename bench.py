@@ -38,8 +38,8 @@ ALGO_BYTES_PER_CELL = 64
 # the counters read a device-to-device copy of known size correctly) and FP64
 # flops (64 lanes x (ADD + MUL + TRANS + 2 FMA) of SQ_INSTS_VALU_*_F64).
 # profiles/fused_step_3ph_256.md holds the runs and their commands.
-TRAFFIC_BYTES_PER_CELL = 75.1
-FP64_FLOPS_PER_CELL = 945.4
+TRAFFIC_BYTES_PER_CELL = 74.8
+FP64_FLOPS_PER_CELL = 872.8
 
 
 def _cpu_model() -> str:
@@ -296,7 +296,7 @@ ENERGY2_ALGO_BYTES_PER_CELL = 56
 # for the three-phase step: HBM bytes (FETCH_SIZE x 2.0 + WRITE_SIZE) and FP64
 # flops; profiles/energy2_2ph_256x256x128.md holds the runs and their commands.
 ENERGY2_TRAFFIC_BYTES_PER_CELL = 71.7
-ENERGY2_FP64_FLOPS_PER_CELL = 1689.2
+ENERGY2_FP64_FLOPS_PER_CELL = 1640.8
 
 
 def bench_energy(nx: int = 256, ny: int = 256, nz: int = 128, steps: int = 20) -> dict:
@@ -452,6 +452,31 @@ def bench_dropin(n: int = 256, warmup: int = 5, steps: int = 50) -> dict:
             "capaz_step_same_deck": leg(res["direct"])}
 
 
+def bench_adjoint_once(n: int = 96, nsteps: int = 50) -> dict:
+    """One gradient of the history-matching workload's first `nsteps` steps
+    under sqrt(nsteps) checkpoints (capaz_adjoint, segment recompute on the
+    second stream), observations at the steps it reaches, after one untimed
+    gradient: a short run of every adjoint kernel, for counter collection."""
+    from capaz_amd import capaz, workloads
+
+    keys, st, lnK_true, lnK0, ob = workloads.history_matching(n, 400)
+    steps = np.array([s for s in ob["steps"] if s <= nsteps])
+    obs = dict(ob, steps=steps, S_w=np.full((len(steps), len(ob["cells"])), 0.3),
+               S_n=np.full((len(steps), len(ob["cells"])), 0.2))
+    d = capaz.Domain(keys, dict(st, K=np.exp(lnK0)))
+    d.adjoint(nsteps, obs, checkpoint_every=-1)
+    for f in ("P_w", "S_w", "S_n", "P_n", "P_g"):
+        d.set_field(f, st[f])
+    d.sync()
+    t0 = time.perf_counter()
+    J, gK, gm = d.adjoint(nsteps, obs, checkpoint_every=-1)
+    d.sync()
+    sec = time.perf_counter() - t0
+    d.close()
+    return {"workload": f"history matching at {n}^3, {nsteps} steps, sqrt(N) checkpoints", "J": J,
+            "seconds": round(sec, 4), "finite": bool(np.isfinite(gK).all() and np.isfinite(gm).all())}
+
+
 def bench_shard(n: int = 256, gpus: int = 8, steps: int = 200, warmup: int = 20) -> dict:
     """One GPU's share of the headline workload under a z-slab decomposition
     over `gpus` GPUs, at its real shape: n x n x (n/gpus + 2) cells (the owned
@@ -583,7 +608,7 @@ def main() -> int:
     ap.add_argument("--no-energy", action="store_true")
     ap.add_argument("--no-shard", action="store_true")
     ap.add_argument("--no-dropin", action="store_true")
-    ap.add_argument("--only", choices=("ensemble", "adjoint", "energy", "shard", "dropin"), default=None,
+    ap.add_argument("--only", choices=("ensemble", "adjoint", "adjoint-once", "energy", "shard", "dropin"), default=None,
                     help="run only this secondary measurement and print it")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device 5 times (a copy of known size, to scale "
@@ -604,8 +629,8 @@ def main() -> int:
         return 2
     if args.only:
         # one secondary measurement alone (for profiling), as its own JSON line
-        fn = {"ensemble": bench_ensemble, "adjoint": bench_adjoint, "energy": bench_energy,
-              "shard": bench_shard, "dropin": bench_dropin}[args.only]
+        fn = {"ensemble": bench_ensemble, "adjoint": bench_adjoint, "adjoint-once": bench_adjoint_once,
+              "energy": bench_energy, "shard": bench_shard, "dropin": bench_dropin}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
         return 0
 

@@ -5,12 +5,14 @@
 // Parity rule: every function below produces the same IEEE bits as the
 // reference CPU path (cpu.cpp, Two-phase/two-phase.cpp,
 // Three-phase/three-phase.cpp, gauss.cpp, energy.cpp) for finite operands in
-// the normal range, up to the sign of an exact zero.  Three transformations
+// the normal range, up to the sign of an exact zero.  Four transformations
 // are used, each exact:
 //   * a / b with a constant b is exact_div(a, b, RN(1/b)): one FMA residual
 //     correction of a*RN(1/b) returns RN(a/b) (Markstein's theorem, Muller et
 //     al., Handbook of Floating-Point Arithmetic, 2nd ed., Thm 4.10);
 //   * x / 2. is x * 0.5 (a power-of-two scaling is exact);
+//   * the upwind parts (x + |x|) / 2 and (x - |x|) / 2 are max(x, 0) and
+//     min(x, 0) (for |x| < 2^1023: x +- |x| is 2x or +-0, exactly);
 //   * terms that are exact zeros in the reference (the CPU path's capillary
 //     pressures and well rates, three-phase.cpp:23-143,529-550) are folded:
 //     x + 0 == x and x - (+-0) == x for x != 0.
