@@ -143,6 +143,7 @@ struct AdjCoefLds {
     double v[12][kTY + 2][kTX + 2];  // P[3], Xi[3], ro[3], mu[3]
 };
 
+template <int CL>
 __device__ __forceinline__ AdjNb adj_stage(const DevParams& p, const AdjIO& a, int i, int j, int k, bool in_grid) {
     AdjNb n;
     n.exists = in_grid;
@@ -152,7 +153,7 @@ __device__ __forceinline__ AdjNb adj_stage(const DevParams& p, const AdjIO& a, i
     if (!in_grid) return n;
     const int64_t q = (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz;
     n.interior = interior3(p, i, j, k);
-    n.c = coef_v(p, a, q, n.interior);
+    n.c = coef_v<CL>(p, a, q, n.interior);
     if (n.interior) {
         n.mu[0] = a.mu_w[q];
         n.mu[1] = a.mu_n[q];
@@ -186,6 +187,7 @@ __device__ __forceinline__ AdjNb lds_get(const DevParams& p, const AdjCoefLds& l
     return n;
 }
 
+template <int CL>
 __global__ void __launch_bounds__(kTX * kTY) k_adj_coef_z(DevParams p, AdjIO a, int kchunk) {
     __shared__ AdjCoefLds lds;
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -202,16 +204,16 @@ __global__ void __launch_bounds__(kTX * kTY) k_adj_coef_z(DevParams p, AdjIO a, 
     const bool halo = hi >= 0 && hi < p.Nx && hj >= 0 && hj < p.Ny;
     const bool hasz = !(p.Nz < 2);
 
-    AdjNb zm = adj_stage(p, a, i, j, k0 - 1, own && hasz && k0 >= 1);
-    AdjNb c = adj_stage(p, a, i, j, k0, own);
+    AdjNb zm = adj_stage<CL>(p, a, i, j, k0 - 1, own && hasz && k0 >= 1);
+    AdjNb c = adj_stage<CL>(p, a, i, j, k0, own);
     for (int k = k0; k <= k1; ++k) {
         if (own) lds_put(lds, ty + 1, tx + 1, c);
-        if (halo) lds_put(lds, hy, hx, adj_stage(p, a, hi, hj, k, true));
+        if (halo) lds_put(lds, hy, hx, adj_stage<CL>(p, a, hi, hj, k, true));
         __syncthreads();
-        const AdjNb zp = adj_stage(p, a, i, j, k + 1, own && hasz && k + 1 < p.Nz);
+        const AdjNb zp = adj_stage<CL>(p, a, i, j, k + 1, own && hasz && k + 1 < p.Nz);
         if (own) {
             const int64_t l = (int64_t)i + (int64_t)j * p.sy + (int64_t)k * p.sz;
-            const CoefD cd = coef_d(p, a, l, c.interior);
+            const CoefD cd = coef_d<CL>(p, a, l, c.interior);
             double lP[3] = {0., 0., 0.}, lL[3] = {0., 0., 0.}, lro[3] = {0., 0., 0.};
             if (!(p.Nx < 2))
                 adj_coef_axis(p, cd, c.mu, lds_get(p, lds, ty + 1, tx, i - 1, j, k),
@@ -225,6 +227,16 @@ __global__ void __launch_bounds__(kTX * kTY) k_adj_coef_z(DevParams p, AdjIO a, 
         zm = c;
         c = zp;
     }
+}
+
+hipError_t launch_adj_coef_z(const DevParams& p, const AdjIO& a, dim3 g, int kchunk, hipStream_t s) {
+    if (p.closure == 0)
+        hipLaunchKernelGGL(k_adj_coef_z<0>, g, dim3(kTX, kTY), 0, s, p, a, kchunk);
+    else if (p.corey_n == 2.)
+        hipLaunchKernelGGL(k_adj_coef_z<2>, g, dim3(kTX, kTY), 0, s, p, a, kchunk);
+    else
+        hipLaunchKernelGGL(k_adj_coef_z<1>, g, dim3(kTX, kTY), 0, s, p, a, kchunk);
+    return hipGetLastError();
 }
 
 // Planes per tile: enough workgroups to fill the GPU several times over,
@@ -274,8 +286,7 @@ hipError_t launch_adjoint_step_tiled(const DevParams& p, const AdjIO& a, hipStre
     const int kc2 = adj_kchunk(p.Nz, txy2);
     const dim3 g2((unsigned)((p.Nx + kTX - 1) / kTX), (unsigned)((p.Ny + kTY - 1) / kTY),
                   (unsigned)((p.Nz + kc2 - 1) / kc2));
-    hipLaunchKernelGGL(k_adj_coef_z, g2, dim3(kTX, kTY), 0, s, p, a, kc2);
-    return hipGetLastError();
+    return launch_adj_coef_z(p, a, g2, kc2, s);
 }
 
 hipError_t launch_adjoint_coef(const DevParams& p, const AdjIO& a, hipStream_t s) {

@@ -331,6 +331,9 @@ int capaz_batch_step(capaz_batch* b, long first_step, long nsteps) {
     int rc;
     if ((rc = check_batch(b))) return rc;
     if (nsteps < 0 || nsteps > (1L << 24)) return bfail(b, CAPAZ_ERR_INVALID, "capaz_batch_step: nsteps out of range");
+    for (capaz_ctx* c : b->dom)
+        if (c->halo_pack)
+            return bfail(b, CAPAZ_ERR_INVALID, "capaz_batch_step: a member has a halo pack, which batched steps do not write");
     return run_steps(b, nsteps, b->use_graph != 0);
 }
 

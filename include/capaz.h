@@ -294,7 +294,10 @@ int capaz_save_faces(capaz_ctx* ctx, const int* fields, int nfields, int axis, c
  * P_w (the value the next step gives them) instead of the previous one.
  * dbuf NULL turns it off.  The energy models, and contexts whose m has a zero
  * in an interior cell, are rejected (CAPAZ_ERR_INVALID, at capaz_step for
- * the latter); Nz must be at least 3. */
+ * the latter); Nz must be at least 3.  Only capaz_step writes the pack:
+ * capaz_batch_step rejects a batch with a member that has one, and the
+ * history-matching calls (capaz_objective, capaz_adjoint, capaz_tangent)
+ * leave it as it is. */
 int capaz_set_halo_pack(capaz_ctx* ctx, double* dbuf);
 
 /* The receiving end of that exchange, with no unpack (no counterpart in the

@@ -518,7 +518,8 @@ def test_halo_pack_equals_load_faces_after_each_step(name):
 
 def test_halo_pack_rejections():
     """The energy models and m == 0 interior cells are rejected (the step
-    takes the entry-point sequence there), as is Nz below 3."""
+    takes the entry-point sequence there), as is Nz below 3, and a batch
+    whose member has a pack."""
     from capaz_amd import hip
     buf = hip.DeviceBuffer(8 * 1024)
     d = capaz.Domain(case("3phe", 5, 5, 5), random_state(case("3phe", 5, 5, 5), 1))
@@ -541,6 +542,15 @@ def test_halo_pack_rejections():
     d.set_halo_pack(0)
     d.run(1)
     d.close()
+    # batched steps do not write a member's pack
+    keys = case("3ph", 6, 5, 5, CLOSURE="corey")
+    b = capaz.Batch(keys, [dict(dims=(6, 5, 5), **random_state(keys, 2))])
+    b.domains[0].set_halo_pack(buf.address)
+    with pytest.raises(RuntimeError, match="halo pack"):
+        b.run(1)
+    b.domains[0].set_halo_pack(0)
+    b.run(1)
+    b.close()
     buf.close()
 
 
