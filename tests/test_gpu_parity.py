@@ -551,6 +551,53 @@ def test_halo_pack_equals_load_faces_after_each_step(name):
     e.close()
 
 
+def test_halo_pack_at_the_shard_shape():
+    """The pack at the shape it is measured on: one z-slab of the 256^3
+    headline over 8 GPUs (256 x 256 x 34).  After each of three steps the
+    pack equals capaz_load_faces of the state (P_n, P_g interior: the new
+    P_w), and the pack taken into the ghost planes through
+    capaz_ghost_planes gives the next step what capaz_save_faces gives it."""
+    import ctypes
+    from capaz_amd import hip
+    keys, st = workloads.headline(256)
+    keys = dict(keys, NZ=34)
+    st = {f: np.ascontiguousarray(v[:34]) for f, v in st.items()}
+    d, e = capaz.Domain(keys, st), capaz.Domain(keys, st)
+    del st
+    names = ("P_w", "S_w", "S_n", "P_n", "P_g")
+    plane = 256 * 256
+    pack, ref = hip.DeviceBuffer(8 * plane * 10), hip.DeviceBuffer(8 * plane * 10)
+    d.set_halo_pack(pack.address)
+    L = capaz.load()
+    ids = (ctypes.c_int * 5)(*[capaz.FIELD_ID[f] for f in names])
+    jj, ii = np.meshgrid(np.arange(256), np.arange(256), indexing="ij")
+    inner = ((jj > 0) & (jj < 255) & (ii > 0) & (ii < 255)).reshape(-1)
+    for step in range(1, 4):
+        d.run(1, first_step=step)
+        e.run(1, first_step=step)
+        assert L.capaz_load_faces(e._ctx, ids, 5, 2, ctypes.c_void_p(ref.address)) == 0
+        d.sync()
+        e.sync()
+        got = pack.download(plane * 10).reshape(5, 2, plane)
+        want = ref.download(plane * 10).reshape(5, 2, plane)
+        want[3:, :, inner] = want[0][None, :, inner]
+        assert bit_equal(got, want), f"step {step}"
+        ghosts = d.ghost_planes()
+        for q in range(10):
+            hip.copy(ghosts[q], pack.address + 8 * plane * q, 8 * plane)
+        ref.upload(got.reshape(-1))
+        assert L.capaz_save_faces(e._ctx, ids, 5, 2, ctypes.c_void_p(ref.address)) == 0
+        e.sync()
+    d.run(1, first_step=4)
+    e.run(1, first_step=4)
+    for f in names:
+        assert bit_equal(d.field(f), e.field(f)), f
+    pack.close()
+    ref.close()
+    d.close()
+    e.close()
+
+
 def test_halo_pack_rejections():
     """The energy models and m == 0 interior cells are rejected (the step
     takes the entry-point sequence there), as is Nz below 3, and a batch
