@@ -62,15 +62,16 @@ CAPAZ_HD void corey_d(const DevParams& p, double s, double& kr, double& dkr) {
         kr = s * s;
         dkr = 2. * s;
     } else {
-        kr = pow(s, p.corey_n);
-        dkr = p.corey_n * pow(s, p.corey_n - 1.);
+        kr = pow_cl(s, p.corey_n);
+        dkr = p.corey_n * pow_cl(s, p.corey_n - 1.);
     }
 }
 
 // The reference's van Genuchten k_r (three-phase.cpp:146-186, as vg_k_w,
 // vg_k_n and vg_k_g evaluate them) and their partial derivatives in the
 // effective saturations x = Swe, y = Sne (z = Sge = 1 - x - y), by the chain
-// rule on each power (a = p.vg_a, b = p.vg_b, c = p.vg_c):
+// rule on each power (a = p.vg_a, b = p.vg_b, c = p.vg_c; the powers are the
+// forward step's pow_cl, whose log of a base the compiler computes once):
 //   k_w = x^(1/2) t^2,           t = 1 - (1 - x^a)^b        (x >= 1e-3)
 //   k_g = z^(1/2) v^c,           v = 1 - (1 - z)^a          (z >= 1e-3)
 //   k_n = y k_nw k_ng / D,       D = (1 - x)(x + y)         (y >= 1e-3)
@@ -81,25 +82,25 @@ CAPAZ_HD void vg_d(const DevParams& p, double x, double y, double& kw, double& k
     const double z = 1. - x - y;
     kw = kn = kg = dkw_dx = dkn_dx = dkn_dy = dkg_dz = 0.;
     if (x >= 1e-3) {
-        const double u = 1. - pow(x, a), t = 1. - pow(u, b);
-        const double dt = b * pow(u, b - 1.) * a * pow(x, a - 1.);
+        const double u = 1. - pow_cl(x, a), t = 1. - pow_cl(u, b);
+        const double dt = b * pow_cl(u, b - 1.) * a * pow_cl(x, a - 1.);
         const double sx = sqrt(x);
         kw = sx * t * t;
         dkw_dx = 0.5 / sx * t * t + 2. * sx * t * dt;
     }
     if (z >= 1e-3) {
-        const double v = 1. - pow(1. - z, a), vc = pow(v, c);
-        const double dv = a * pow(1. - z, a - 1.);
+        const double v = 1. - pow_cl(1. - z, a), vc = pow_cl(v, c);
+        const double dv = a * pow_cl(1. - z, a - 1.);
         const double sz = sqrt(z);
         kg = sz * vc;
-        dkg_dz = 0.5 / sz * vc + sz * c * pow(v, c - 1.) * dv;
+        dkg_dz = 0.5 / sz * vc + sz * c * pow_cl(v, c - 1.) * dv;
     }
     if (y >= 1e-3) {
-        const double w = 1. - pow(x, a), sw = sqrt(1. - x);
-        const double knw = sw * pow(w, c);
-        const double dknw = -0.5 / sw * pow(w, c) - sw * c * pow(w, c - 1.) * a * pow(x, a - 1.);
-        const double q = 1. - pow(y, a), s = 1. - pow(q, b);
-        const double ds = b * pow(q, b - 1.) * a * pow(y, a - 1.);
+        const double w = 1. - pow_cl(x, a), sw = sqrt(1. - x);
+        const double knw = sw * pow_cl(w, c);
+        const double dknw = -0.5 / sw * pow_cl(w, c) - sw * c * pow_cl(w, c - 1.) * a * pow_cl(x, a - 1.);
+        const double q = 1. - pow_cl(y, a), s = 1. - pow_cl(q, b);
+        const double ds = b * pow_cl(q, b - 1.) * a * pow_cl(y, a - 1.);
         const double sy = sqrt(y);
         const double kng = sy * s * s;
         const double dkng = 0.5 / sy * s * s + 2. * sy * s * ds;
