@@ -42,7 +42,7 @@ __global__ void __launch_bounds__(TX * TY)
         const int lt = t - d->tile_begin;
         const int bx = lt % d->tiles_x, r = lt / d->tiles_x;
         fused_z_tile<MODEL, CLOSURE, TX, TY, true>(d->p, d->io[parity], kchunk, bx, r % d->tiles_y, r / d->tiles_y,
-                                                   lds, FusedStore<MODEL, kFusedFaces<MODEL>>{d->io[parity]});
+                                                   lds, FusedStore<MODEL, kFusedFaces<MODEL>, CLOSURE>{d->io[parity]});
         if (!counter) return;
     }
 }

@@ -130,32 +130,33 @@ CAPAZ_HD int64_t inward(const DevParams& p, int i, int j, int k) {
 
 // x^y in the reference closures (the van Genuchten and Brooks-Corey k_r, the
 // energy model's mu_g(T) and lambda_g(T), a Corey exponent read at run time).
-// On the device: exp(y * log(x)) from the device library's log and exp, about
-// 130 VALU instructions against some 215 for its pow, with log(x) computed once
-// where powers share a base (pow_base / pow_of).  The rounded log puts
+// On the device: exp(y * log|x|) from the device library's log and exp, about
+// 130 VALU instructions against some 215 for its pow, with the log computed
+// once where powers share a base (pow_base / pow_of).  The rounded log puts
 // |y ln x| ulp into the exponent, so the two agree to a few ulp for these
 // bases and exponents -- within the tolerance the reference closures are held
-// to, since the device library's pow is not glibc's either.  A negative base
-// or a zero exponent takes the device library's pow, out of line (the closures
-// reach it only for states outside their range).  The host build keeps glibc's
-// pow, whose bits the reference's golden vectors hold.
-#if defined(__HIP_DEVICE_COMPILE__)
-__device__ __attribute__((noinline)) inline double pow_outline(double x, double y) { return pow(x, y); }
-#endif
+// to, since the device library's pow is not glibc's either.  pow's other
+// cases, which the closures reach only for states outside their range, by
+// selects (no call, which would cost registers in every kernel that holds a
+// closure's code): a zero exponent gives 1, a negative finite base gives
+// +-|x|^y for an integer exponent and NaN otherwise.  The host build keeps
+// glibc's pow, whose bits the reference's golden vectors hold.
 struct PowBase {
-    double x, lx;  // the base and (device) its log
+    double x, lx;  // the base and (device) log|x|
 };
 CAPAZ_HD PowBase pow_base(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return {x, log(x)};
+    return {x, log(fabs(x))};
 #else
     return {x, 0.};
 #endif
 }
 CAPAZ_HD double pow_of(const PowBase& b, double y) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    if (__builtin_expect(b.x < 0. || y == 0., 0)) return pow_outline(b.x, y);
-    return exp(y * b.lx);
+    double r = exp(y * b.lx);
+    const bool integer = y == trunc(y), odd = integer && 0.5 * y != trunc(0.5 * y);
+    r = b.x < 0. ? (integer ? (odd ? -r : r) : __builtin_nan("")) : r;
+    return y == 0. ? 1. : r;
 #else
     return pow(b.x, y);
 #endif
@@ -478,10 +479,13 @@ CAPAZ_HD int reverse3(double* a) {
 // ---- Newton solves ------------------------------------------------------------------------
 
 // Two-phase fixed-iteration 2x2 Newton by Cramer's rule (two-phase.cpp:25-56).
+// CL: the fused kernels' closure id (0 the reference's Brooks-Corey, 1 and 2
+// Corey), or -1 for p.closure at run time.
+template <int CL = -1>
 CAPAZ_HD void newton2(const DevParams& p, double& P, double& Sn, double roS_w, double roS_n) {
     for (int w = 1; w <= p.iterations; w++) {
         double P_k, PkS;
-        if (p.closure == 1) {
+        if (CL >= 1 || (CL < 0 && p.closure == 1)) {
             P_k = 0.;
             PkS = 0.;
         } else {

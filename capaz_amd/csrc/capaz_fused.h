@@ -388,8 +388,9 @@ __device__ __forceinline__ void store_faces(const DevParams& p, const StepIO& io
 // k_fused_faces writes them after the kernel).  The two-phase kernels write
 // their faces here (the strided x faces cost a separate pass more than the
 // epilogue costs); in the three-phase kernels the extra registers would cost
-// a wave per SIMD, so they keep the face pass.
-template <int MODEL, bool FACES = false>
+// a wave per SIMD, so they keep the face pass.  CLOSURE: the kernel's
+// (the two-phase Newton then carries only its closure's code).
+template <int MODEL, bool FACES = false, int CLOSURE = -1>
 struct FusedStore {
     StepIO io;
     __device__ __forceinline__ void operator()(const DevParams& p, int i, int j, int k, int64_t l,
@@ -400,7 +401,7 @@ struct FusedStore {
         if constexpr (MODEL == 2) {
             const double A1 = c.ro[0] * (1. - Sn) - dtm * div[0];
             const double A2 = c.ro[1] * Sn - dtm * div[1];
-            newton2(p, Pw, Sn, A1, A2);
+            newton2<CLOSURE>(p, Pw, Sn, A1, A2);
             io.Pw1[l] = Pw;
             io.Sn1[l] = Sn;
             if constexpr (FACES) store_faces<2>(p, io, i, j, k, l, c, Pw, 0., Sn);
@@ -424,9 +425,9 @@ constexpr bool kFusedFaces = MODEL == 2;
 
 // FusedStore that also records the cell's flux divergence div_a (field a at
 // div + a*stride), for the adjoint's backward Newton.
-template <int MODEL>
+template <int MODEL, int CLOSURE = -1>
 struct FusedStoreRecord {
-    FusedStore<MODEL, kFusedFaces<MODEL>> step;
+    FusedStore<MODEL, kFusedFaces<MODEL>, CLOSURE> step;
     double* div;
     int64_t stride;
     __device__ __forceinline__ void operator()(const DevParams& p, int i, int j, int k, int64_t l,
@@ -613,8 +614,8 @@ __global__ void __launch_bounds__(TX * TY) k_fused_z_record(DevParams p, StepIO 
     int bx, by, bz;
     xcd_tile(bx, by, bz);
     fused_z_tile<MODEL, CLOSURE, TX, TY, CARRY>(p, io, kchunk, bx, by, bz, lds,
-                                                FusedStoreRecord<MODEL>{FusedStore<MODEL, kFusedFaces<MODEL>>{io},
-                                                                        div, stride});
+                                                FusedStoreRecord<MODEL, CLOSURE>{
+                                                    FusedStore<MODEL, kFusedFaces<MODEL>, CLOSURE>{io}, div, stride});
 }
 
 
@@ -624,7 +625,7 @@ __global__ void __launch_bounds__(TX * TY) k_fused_z(DevParams p, StepIO io, int
     int bx, by, bz;
     xcd_tile(bx, by, bz);
     fused_z_tile<MODEL, CLOSURE, TX, TY, CARRY>(p, io, kchunk, bx, by, bz, lds,
-                                                FusedStore<MODEL, kFusedFaces<MODEL>>{io});
+                                                FusedStore<MODEL, kFusedFaces<MODEL>, CLOSURE>{io});
 }
 
 // Cell number t of a box of ni x nj cells per plane starting at (i0, j0, k0),
