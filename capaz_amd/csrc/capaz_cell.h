@@ -132,10 +132,10 @@ CAPAZ_HD int64_t inward(const DevParams& p, int i, int j, int k) {
 // energy model's mu_g(T) and lambda_g(T), a Corey exponent read at run time).
 // On the device: exp(y * log|x|) from the device library's log and exp, about
 // 130 VALU instructions against some 215 for its pow, with the log computed
-// once where powers share a base (pow_base / pow_of).  The rounded log puts
-// |y ln x| ulp into the exponent, so the two agree to a few ulp for these
-// bases and exponents -- within the tolerance the reference closures are held
-// to, since the device library's pow is not glibc's either.  pow's other
+// once where powers share a base (pow_base / pow_of).  The roundings of log x
+// and of y * log x put up to 2 |y ln x| ulp into the result (tens of ulp for
+// the closures' bases, x >= 1e-15), within the tolerance the reference
+// closures are held to, since the device library's pow is not glibc's either.  pow's other
 // cases, which the closures reach only for states outside their range, by
 // selects (no call, which would cost registers in every kernel that holds a
 // closure's code): a zero exponent gives 1, a negative finite base gives

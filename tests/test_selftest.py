@@ -3,7 +3,8 @@ operations they replace (tests/hip/capaz_selftest.hip): the reciprocal used
 for Gauss-Jordan pivots and the exact division built on it, over random
 operands including all-ones significands and powers of two, and the square
 root and square that stand for pow(x, 0.5) and pow(t, 2.) in the van
-Genuchten closures."""
+Genuchten closures, and exp(y * log|x|), which stands for the closures' other
+powers."""
 import ctypes
 import os
 
@@ -73,3 +74,55 @@ def test_pow_two_is_the_rounded_square():
     nan = np.isnan(want)
     assert np.array_equal(np.isnan(got), nan)
     assert np.array_equal(got[~nan].view(np.uint64), want[~nan].view(np.uint64))
+
+
+def device_pow(x, e):
+    from capaz_amd import capaz
+    capaz.load()
+    L = ctypes.CDLL(LIB)
+    L.capaz_selftest_pow.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_ulonglong]
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    assert L.capaz_selftest_pow(x.ctypes.data, e, y.ctypes.data, x.size) == 0
+    return y
+
+
+# The closures' exponents: the reference Three-phase deck's van Genuchten
+# a = A/(A-1), b = (A-1)/A, c = 2(A-1)/A with A = 3.25; mu_g(T)'s 0.683; a
+# Corey exponent; Brooks-Corey-type exponents; integers.
+POW_EXPONENTS = [3.25 / 2.25, 2.25 / 3.25, 4.5 / 3.25, 0.683, 2.5, 3.0 + 2.0 / 2.7, 1.0 / 2.7, 2.0, 3.0, 0.5]
+
+
+@pytest.mark.parametrize("e", POW_EXPONENTS)
+def test_closure_power_is_within_a_few_ulp_of_pow(e):
+    """pow_cl(x, e) = exp(e * log|x|) on the device against numpy's pow, on
+    the closures' bases: within 2 (|e ln x| + 1) + 1 ulp -- the roundings of
+    log x and of e * log x are absolute errors of up to |e ln x| 2^-53 each in
+    the exponent, which exp turns into relative ones; exp(e * log(x)) from
+    numpy's correctly rounded log and exp reaches 1.8 (|e ln x| + 1) on these
+    operands -- and pow's special cases exactly: 1 for
+    x = 1, 0 for x = 0, NaN for a NaN or a negative base with a non-integer
+    exponent, +-|x|^e for a negative base with an integer one."""
+    x = closure_operands(1 << 20, 7)
+    x = x[np.isfinite(x) & (x > 0)]
+    got = device_pow(x, e)
+    want = np.power(x, e)
+    ok = want > 0
+    ulps = np.abs(got[ok] - want[ok]) / np.spacing(want[ok])
+    bound = 2.0 * (np.abs(e * np.log(x[ok])) + 1.0) + 1.0
+    assert (ulps <= bound).all(), (e, float(ulps.max()), float(x[ok][np.argmax(ulps - bound)]))
+    special = np.array([1.0, 0.0, -0.0, np.nan, -2.0, -0.5, np.inf])
+    gs = device_pow(special, e)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ws = np.power(special, e)
+    assert gs[0] == 1.0 and gs[1] == 0.0 and gs[2] == 0.0 and np.isnan(gs[3]) and gs[6] == np.inf
+    for k in (4, 5):
+        if e == int(e):
+            assert np.sign(gs[k]) == np.sign(ws[k]) and abs(gs[k] - ws[k]) <= 4 * np.spacing(abs(ws[k])), (k, gs[k], ws[k])
+        else:
+            assert np.isnan(gs[k])
+
+
+def test_closure_power_with_a_zero_exponent_is_one():
+    x = np.array([0.0, 1.0, 0.3, -2.0, np.nan, np.inf])
+    assert (device_pow(x, 0.0) == 1.0).all()
