@@ -141,12 +141,53 @@ CAPAZ_HD int64_t inward(const DevParams& p, int i, int j, int k) {
 // closure's code): a zero exponent gives 1, a negative finite base gives
 // +-|x|^y for an integer exponent and NaN otherwise.  The host build keeps
 // glibc's pow, whose bits the reference's golden vectors hold.
+#if defined(__HIP_DEVICE_COMPILE__)
+// ln x for x > 0 (pow_base's log|x|): x = 2^k m with m in [sqrt(1/2),
+// sqrt(2)) by the hardware's frexp (subnormals included), then
+// ln m = 2 atanh(s), s = (m - 1) / (m + 1) (|s| <= 0.1716; m - 1 is exact),
+// by the series 2 s (1 + s^2/3 + s^4/5 + ... + s^20/21), whose next term is
+// below 1e-17 of the sum, and k ln 2 with ln 2 split so that k * LN2_HI is
+// exact.  s carries ~2^-52 relative error (the rounded m + 1 and a reciprocal
+// with two Newton steps), which the result keeps for x near 1; for other x
+// the final additions' rounding dominates, as in a correctly rounded log.
+// 0 gives -inf, +inf gives +inf, NaN and negative x give NaN.
+__device__ __forceinline__ double log_cl(double x) {
+    constexpr double kLn2Hi = 0x1.62e42fefa3800p-1, kLn2Lo = 0x1.ef35793c76730p-45;
+    double m = __builtin_amdgcn_frexp_mant(x);  // [0.5, 1) for finite x > 0
+    int k = __builtin_amdgcn_frexp_exp(x);
+    const bool low = m < 0x1.6a09e667f3bcdp-1;  // sqrt(1/2)
+    m = low ? 2. * m : m;
+    k = low ? k - 1 : k;
+    const double q = m + 1.;
+    double r = __builtin_amdgcn_rcp(q);
+    r = fma(fma(-q, r, 1.), r, r);
+    r = fma(fma(-q, r, 1.), r, r);
+    const double s = (m - 1.) * r, z = s * s;
+    double poly = 1. / 21.;
+    poly = fma(poly, z, 1. / 19.);
+    poly = fma(poly, z, 1. / 17.);
+    poly = fma(poly, z, 1. / 15.);
+    poly = fma(poly, z, 1. / 13.);
+    poly = fma(poly, z, 1. / 11.);
+    poly = fma(poly, z, 1. / 9.);
+    poly = fma(poly, z, 1. / 7.);
+    poly = fma(poly, z, 1. / 5.);
+    poly = fma(poly, z, 1. / 3.);
+    const double lm = 2. * fma(s * z, poly, s);  // ln m
+    const double kd = (double)k;
+    double l = fma(kd, kLn2Hi, fma(kd, kLn2Lo, lm));
+    l = x == 0. ? -__builtin_inf() : l;
+    l = x == __builtin_inf() ? __builtin_inf() : l;
+    return x > 0. || x == 0. ? l : __builtin_nan("");  // NaN and x < 0: NaN
+}
+#endif
+
 struct PowBase {
     double x, lx;  // the base and (device) log|x|
 };
 CAPAZ_HD PowBase pow_base(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return {x, log(fabs(x))};
+    return {x, log_cl(fabs(x))};
 #else
     return {x, 0.};
 #endif

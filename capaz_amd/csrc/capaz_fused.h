@@ -619,13 +619,14 @@ __global__ void __launch_bounds__(TX * TY) k_fused_z_record(DevParams p, StepIO 
 }
 
 
-// Occupancy bound: the three-phase kernels with a closure that raises powers
-// (the reference's van Genuchten k_r, a Corey exponent read at run time) need
-// ~160 VGPRs, which leaves one 512-thread workgroup per CU; held to 128 (4
-// waves per SIMD) they spill some 32 VGPRs and still run faster (the
-// reference's deck at 256^3: 1.76 against 1.84 ms per step).
+// Occupancy bound: the kernels with a closure that raises powers (the
+// reference's van Genuchten or Brooks-Corey k_r, a Corey exponent read at run
+// time) need 130-160 VGPRs, which leaves one 512-thread workgroup per CU; held
+// to 128 (4 waves per SIMD) they spill at most some 32 VGPRs and still run
+// faster (the reference's three-phase deck at 256^3: 1.76 against 1.84 ms per
+// step).
 template <int MODEL, int CLOSURE>
-constexpr int kFusedWaves = MODEL == 3 && (CLOSURE == 0 || CLOSURE == 1) ? 4 : 1;
+constexpr int kFusedWaves = CLOSURE == 0 || CLOSURE == 1 ? 4 : 1;
 
 template <int MODEL, int CLOSURE, int TX, int TY, bool CARRY>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(kFusedWaves<MODEL, CLOSURE>)))

@@ -55,6 +55,13 @@ __global__ void k_div(uint64_t n, uint64_t seed, int emin, int emax, unsigned lo
     if (local) atomicAdd(bad, local);
 }
 
+__global__ void k_log(const double* x, double* y, uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (i < n) y[i] = capaz::log_cl(x[i]);
+#endif
+}
+
 __global__ void k_pow(const double* x, double e, double* y, uint64_t n) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = capaz::pow_cl(x[i], e);
@@ -77,6 +84,22 @@ int capaz_selftest_map(int which, const double* x, double* y, unsigned long long
     if (hipMalloc(&dx, n * sizeof(double)) == hipSuccess && hipMalloc(&dy, n * sizeof(double)) == hipSuccess &&
         hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice) == hipSuccess) {
         hipLaunchKernelGGL(k_map, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, which, dx, dy, (uint64_t)n);
+        if (hipGetLastError() == hipSuccess && hipMemcpy(y, dy, n * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess)
+            rc = 0;
+    }
+    if (dx) (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    return rc;
+}
+
+// y = log_cl(x) on the device (the closures' powers' log) for n host doubles;
+// 0, or -1 on a HIP error.
+int capaz_selftest_log(const double* x, double* y, unsigned long long n) {
+    double *dx = nullptr, *dy = nullptr;
+    int rc = -1;
+    if (hipMalloc(&dx, n * sizeof(double)) == hipSuccess && hipMalloc(&dy, n * sizeof(double)) == hipSuccess &&
+        hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice) == hipSuccess) {
+        hipLaunchKernelGGL(k_log, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, dx, dy, (uint64_t)n);
         if (hipGetLastError() == hipSuccess && hipMemcpy(y, dy, n * sizeof(double), hipMemcpyDeviceToHost) == hipSuccess)
             rc = 0;
     }

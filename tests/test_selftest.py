@@ -126,3 +126,29 @@ def test_closure_power_is_within_a_few_ulp_of_pow(e):
 def test_closure_power_with_a_zero_exponent_is_one():
     x = np.array([0.0, 1.0, 0.3, -2.0, np.nan, np.inf])
     assert (device_pow(x, 0.0) == 1.0).all()
+
+
+def test_closure_log_is_within_two_ulp_of_log():
+    """log_cl, the log of the closures' powers, against numpy's log: within 2
+    ulp (the series' argument s carries ~2^-52 relative error, which shows
+    where ln x is small), on closure operands, normal and subnormal numbers
+    across the range, and exactly for the special values."""
+    from capaz_amd import capaz
+    capaz.load()
+    L = ctypes.CDLL(LIB)
+    L.capaz_selftest_log.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_ulonglong]
+    rng = np.random.default_rng(9)
+    wide = np.exp(rng.uniform(np.log(5e-324), np.log(1.7e308), 1 << 20))
+    x = np.concatenate([closure_operands(1 << 20, 8), wide])
+    x = np.ascontiguousarray(x[np.isfinite(x) & (x > 0)])
+    y = np.empty_like(x)
+    assert L.capaz_selftest_log(x.ctypes.data, y.ctypes.data, x.size) == 0
+    want = np.log(x)
+    nz = want != 0
+    ulps = np.abs(y[nz] - want[nz]) / np.spacing(np.abs(want[nz]))
+    assert ulps.max() <= 2.0, (float(ulps.max()), float(x[nz][np.argmax(ulps)]))
+    assert (y[~nz] == 0).all()
+    special = np.array([0.0, -0.0, np.inf, -1.0, np.nan])
+    ys = np.empty_like(special)
+    assert L.capaz_selftest_log(special.ctypes.data, ys.ctypes.data, special.size) == 0
+    assert ys[0] == -np.inf and ys[1] == -np.inf and ys[2] == np.inf and np.isnan(ys[3]) and np.isnan(ys[4])
