@@ -240,14 +240,33 @@ __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& i
     const int64_t hcol = (int64_t)hi + (int64_t)hj * p.sy;
     const bool own_ij = inner_ij<MODEL>(p, i, j), halo_ij = inner_ij<MODEL>(p, hi, hj);
 
+    // The z window: with ZPARTS the minus z face is carried between planes
+    // as its two parts of directed(), zneg = neg_part(x1) and zt2 = t2 (x1 =
+    // -dz; computed from the same operands as at the plane itself, one plane
+    // earlier): two values per phase instead of dz and the plane below's Xi
+    // and ro, and no register copies of those.  That frees registers in the
+    // Corey-2 kernels (the headline's: 543 -> 536 us at 256^3), but adds
+    // spill to the kernels already held to 128 VGPRs (the closures with
+    // powers), which keep dz, Xi and ro.
+    constexpr bool ZPARTS = CLOSURE == 2;
     CoefT<NPH> zm{}, c{}, zp{};
-    double dz[NPH];
+    double zneg[NPH], zt2[NPH], dz[NPH];
+#pragma unroll
+    for (int a = 0; a < NPH; ++a) zneg[a] = zt2[a] = dz[a] = 0.;
     if (own_in_grid) {
         c = coef_at<MODEL, CLOSURE>(p, io, col + (int64_t)k0 * p.sz, own_ij && inner_k(p, k0));
         if (hasz) {
             zm = coef_at<MODEL, CLOSURE>(p, io, col + (int64_t)(k0 - 1) * p.sz, own_ij && inner_k(p, k0 - 1));
 #pragma unroll
-            for (int a = 0; a < NPH; ++a) dz[a] = exact_div(c.P[a] - zm.P[a], p.hz, p.r_hz);
+            for (int a = 0; a < NPH; ++a) {
+                const double d = exact_div(c.P[a] - zm.P[a], p.hz, p.r_hz);
+                if constexpr (ZPARTS) {
+                    zneg[a] = neg_part(-1. * d);
+                    zt2[a] = directed_t2(-1. * d, zm.Xi[a], zm.ro[a]);
+                } else {
+                    dz[a] = d;
+                }
+            }
         }
     }
 
@@ -276,9 +295,10 @@ __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& i
 
         if (own_in_grid && hasz)
             zp = coef_at<MODEL, CLOSURE>(p, io, col + (int64_t)(k + 1) * p.sz, own_ij && inner_k(p, k + 1));
-        double dzp[NPH];
+        // the plus z face's difference; the next plane's dz
+        double zd[NPH];
 #pragma unroll
-        for (int a = 0; a < NPH; ++a) dzp[a] = hasz ? exact_div(zp.P[a] - c.P[a], p.hz, p.r_hz) : 0.;
+        for (int a = 0; a < NPH; ++a) zd[a] = hasz ? exact_div(zp.P[a] - c.P[a], p.hz, p.r_hz) : 0.;
 
         if (active) {
             const int64_t l = col + (int64_t)k * p.sz;
@@ -287,8 +307,14 @@ __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& i
             for (int a = 0; a < NPH; ++a) {
                 // order as cpu.cpp:651-708: z, x, y, then (-q + fx + fy + fz)
                 double fz = 0., fx = 0.;
-                if (hasz)
-                    fz = flux_x(dz[a], dzp[a], c.Xi[a], c.ro[a], zm.Xi[a], zm.ro[a], zp.Xi[a], zp.ro[a], p.hz, p.r_hz);
+                if (hasz) {
+                    if constexpr (ZPARTS)
+                        fz = directed_parts(zneg[a], zt2[a], -1. * zd[a], c.Xi[a], c.ro[a], zp.Xi[a], zp.ro[a], p.hz,
+                                            p.r_hz);
+                    else
+                        fz = flux_x(dz[a], zd[a], c.Xi[a], c.ro[a], zm.Xi[a], zm.ro[a], zp.Xi[a], zp.ro[a], p.hz,
+                                    p.r_hz);
+                }
                 if (hasx) {
                     const double Pm = lds.v[a][ty + 1][tx], Pp = lds.v[a][ty + 1][tx + 2];
                     const double dm = exact_div(c.P[a] - Pm, p.hx, p.r_hx);
@@ -304,6 +330,15 @@ __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& i
                                          lds.v[2 * NPH + a][ty + 2][tx + 1]);
                 double s = (a == 2 ? 0. + fx : fx) + fy;
                 div[a] = s + fz;
+            }
+            // the plus z face's parts, the next plane's minus face's (only
+            // active cells read them)
+            if constexpr (ZPARTS) {
+#pragma unroll
+                for (int a = 0; a < NPH; ++a) {
+                    zneg[a] = neg_part(-1. * zd[a]);
+                    zt2[a] = directed_t2(-1. * zd[a], c.Xi[a], c.ro[a]);
+                }
             }
             if constexpr (EpiWantsU<Epi>::value) {
                 // interior cell: central differences on every axis longer than 2
@@ -326,9 +361,11 @@ __device__ __forceinline__ void fused_z_tile(const DevParams& p, const StepIO& i
         // roll the z window
 #pragma unroll
         for (int a = 0; a < NPH; ++a) {
-            dz[a] = dzp[a];
-            zm.Xi[a] = c.Xi[a];
-            zm.ro[a] = c.ro[a];
+            if constexpr (!ZPARTS) {
+                dz[a] = zd[a];
+                zm.Xi[a] = c.Xi[a];
+                zm.ro[a] = c.ro[a];
+            }
             if constexpr (EpiWantsU<Epi>::value) zm.P[a] = c.P[a];
         }
         if (CARRY) c = zp;
