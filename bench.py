@@ -38,8 +38,8 @@ ALGO_BYTES_PER_CELL = 64
 # the counters read a device-to-device copy of known size correctly) and FP64
 # flops (64 lanes x (ADD + MUL + TRANS + 2 FMA) of SQ_INSTS_VALU_*_F64).
 # profiles/fused_step_3ph_256.md holds the runs and their commands.
-TRAFFIC_BYTES_PER_CELL = 74.8
-FP64_FLOPS_PER_CELL = 872.8
+TRAFFIC_BYTES_PER_CELL = 74.7
+FP64_FLOPS_PER_CELL = 873.1
 
 
 def _cpu_model() -> str:
