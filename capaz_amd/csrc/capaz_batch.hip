@@ -19,8 +19,16 @@ namespace capaz {
 
 namespace {
 
+// Occupancy bound: the two-phase Corey-2 tiles of 256 threads (the 32 x 8
+// and 64 x 4 shapes) fit in 94 VGPRs without a spill, 5 waves per SIMD
+// against 4 at the compiler's own 106-108: the 64-realization ensemble went
+// from 46.4 to 50.2 Gcell-steps/s; bound to 6 waves (80 VGPRs, 17 spilled)
+// it ran at 44.5.
 template <int MODEL, int CLOSURE, int TX, int TY>
-__global__ void __launch_bounds__(TX * TY)
+constexpr int kBatchWaves = MODEL == 2 && CLOSURE == 2 && TX * TY == 256 ? 5 : 1;
+
+template <int MODEL, int CLOSURE, int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(kBatchWaves<MODEL, CLOSURE, TX, TY>)))
     k_batch_z(const BatchDomain* __restrict__ doms, const int* __restrict__ owner, int ntiles, int parity,
               int kchunk, int* counter) {
     __shared__ FusedLds<3 * MODEL, TX, TY> lds;
