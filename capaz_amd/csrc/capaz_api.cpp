@@ -517,7 +517,7 @@ int capaz_tune(capaz_ctx* c, const char* key, int value) {
         c->tuning.check_every = value;
     else if (k == "fused_kernel" && (value >= 0 && value <= 2))
         c->tuning.fused_kernel = value;
-    else if (k == "tile_y" && (value == 4 || value == 8))
+    else if (k == "tile_y" && (value == 0 || value == 4 || value == 8))
         c->tuning.tile_y = value;
     else if (k == "kchunk" && value >= 0 && value <= 1 << 20)
         c->tuning.kchunk = value;

@@ -664,9 +664,15 @@ __global__ void __launch_bounds__(TX * TY) k_fused_z_record(DevParams p, StepIO 
 // step).
 template <int MODEL, int CLOSURE>
 constexpr int kFusedWaves = CLOSURE == 0 || CLOSURE == 1 ? 4 : 1;
+// The two-phase Corey-2 tiles of 64 x 4 (256 threads) fit in 94 VGPRs
+// without a spill at 5 waves per SIMD: 330 us per step at 256^3 against 374
+// for 64 x 8 tiles at 4 (the automatic tile_y of that step).  The
+// three-phase ones would spill 28 VGPRs there and take 871 us against 543.
+template <int MODEL, int CLOSURE, int TY>
+constexpr int kFusedWavesTY = MODEL == 2 && CLOSURE == 2 && TY == 4 ? 5 : kFusedWaves<MODEL, CLOSURE>;
 
 template <int MODEL, int CLOSURE, int TX, int TY, bool CARRY>
-__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(kFusedWaves<MODEL, CLOSURE>)))
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(kFusedWavesTY<MODEL, CLOSURE, TY>)))
 k_fused_z(DevParams p, StepIO io, int kchunk) {
     __shared__ FusedLds<3 * MODEL, TX, TY> lds;
     int bx, by, bz;

@@ -350,7 +350,9 @@ hipError_t launch_z(const DevParams& p, const StepIO& io, int kchunk, hipStream_
 template <int MODEL, int CLOSURE>
 hipError_t launch_z_ty(const DevParams& p, const StepIO& io, const Tuning& t, hipStream_t s, const KernelEvents* ev) {
     const bool carry = t.fused_kernel == 1;
-    if (t.tile_y == 8)
+    // tile_y 0: 64 x 4 tiles for the two-phase Corey-2 step, 64 x 8 otherwise
+    const int ty = t.tile_y ? t.tile_y : (MODEL == 2 && CLOSURE == 2 ? 4 : 8);
+    if (ty == 8)
         return carry ? launch_z<MODEL, CLOSURE, 8, true>(p, io, t.kchunk, s, ev)
                      : launch_z<MODEL, CLOSURE, 8, false>(p, io, t.kchunk, s, ev);
     return carry ? launch_z<MODEL, CLOSURE, 4, true>(p, io, t.kchunk, s, ev)

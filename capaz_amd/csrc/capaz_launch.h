@@ -32,7 +32,7 @@ enum class Entry { S, P_Xi_ro, U, RoS, HE, E, Newton, Boundary };
 struct Tuning {
     int check_every = 0;  // non-finite sentinel after every check_every-th step (0: off)
     int fused_kernel = 1;
-    int tile_y = 8;
+    int tile_y = 0;  // 0: automatic (4 for the two-phase Corey-2 step, else 8)
     int kchunk = 0;
     int adjoint_kernel = 2;
     int adjoint_streams = 2;

@@ -189,7 +189,7 @@ int capaz_step(capaz_ctx* ctx, long first_step, long nsteps);
  *   "fused_kernel"  0: one thread per cell; 1: z-marching 64 x tile_y tiles
  *                   with the plane's coefficients staged in LDS (default);
  *                   2: as 1, recomputing the centre coefficients per plane
- *   "tile_y"        4 or 8 (default 8)
+ *   "tile_y"        0, 4 or 8 (default 0: 4 for the two-phase Corey-2 step, 8 otherwise)
  *   "kchunk"        planes per workgroup along z; 0 (default): chosen from
  *                   the grid (16 on large grids, fewer on small ones)
  *   "adjoint_kernel" capaz_adjoint's passes: 2 (default) the backward
