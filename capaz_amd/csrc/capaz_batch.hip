@@ -19,13 +19,18 @@ namespace capaz {
 
 namespace {
 
-// Occupancy bound: the two-phase Corey-2 tiles of 256 threads (the 32 x 8
+// Occupancy bounds: the two-phase Corey-2 tiles of 256 threads (the 32 x 8
 // and 64 x 4 shapes) fit in 94 VGPRs without a spill, 5 waves per SIMD
 // against 4 at the compiler's own 106-108: the 64-realization ensemble went
 // from 46.4 to 50.2 Gcell-steps/s; bound to 6 waves (80 VGPRs, 17 spilled)
-// it ran at 44.5.
+// it ran at 44.5.  The three-phase Corey-2 tiles fit in 128 VGPRs without a
+// spill against the compiler's own 129-131, 4 waves per SIMD against 3 (one
+// 512-thread workgroup per CU for 64 x 8): a 24-realization three-phase
+// batch (edges 32-96) 0.331 -> 0.305 ms per step on 32 x 8 tiles, 0.524 ->
+// 0.376 ms on 64 x 8.
 template <int MODEL, int CLOSURE, int TX, int TY>
-constexpr int kBatchWaves = MODEL == 2 && CLOSURE == 2 && TX * TY == 256 ? 5 : 1;
+constexpr int kBatchWaves =
+    MODEL == 2 && CLOSURE == 2 && TX * TY == 256 ? 5 : (MODEL == 3 && CLOSURE == 2 ? 4 : 1);
 
 template <int MODEL, int CLOSURE, int TX, int TY>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(kBatchWaves<MODEL, CLOSURE, TX, TY>)))
