@@ -155,6 +155,42 @@ def test_random_state_vs_oracle(model, dims, over, exact):
             check_fields(gf, want, tol, f"after {s} steps")
 
 
+LAUNCH_SHAPES = [dict(), dict(tile_y=8), dict(tile_y=4), dict(fused_kernel=2), dict(fused_kernel=2, tile_y=4),
+                 dict(fused_kernel=0), dict(kchunk=3)]
+SHAPE_CASES = [("2ph", dict(CLOSURE="corey", S_WR_0=0.1, S_NR_0=0.05, SOURCE=3, DT=1e-4)),
+               ("3ph", dict(CLOSURE="corey", S_WR_0=0.1, S_NR_0=0.05, SOURCE=1, DT=1e-4)),
+               ("2ph", dict(DT=1e-4, SOURCE=4)), ("3ph", dict(DT=1e-4))]
+
+
+@pytest.mark.parametrize("model,over", SHAPE_CASES,
+                         ids=[f"{m}-{'corey' if o.get('CLOSURE') else 'ref'}" for m, o in SHAPE_CASES])
+def test_every_launch_shape_gives_the_same_bits(model, over):
+    """The fused step's launch shapes (tile_y 4 / 8 and the automatic one,
+    which is 4 for the two-phase Corey-2 step and 8 otherwise; the carried
+    or recomputed centre coefficients; one thread per cell; a fixed chunk
+    of planes) give the same bits after 6 steps on a grid wider and taller
+    than one tile, and, with the Corey closure, the oracle's."""
+    keys = case(model, 70, 13, 11, **over)
+    st = random_state(keys, seed=31)
+    runs = []
+    for tune in LAUNCH_SHAPES:
+        d = capaz.Domain(keys, st)
+        d.tune(**tune)
+        d.run(6)
+        runs.append((tune, d.fields()))
+        d.close()
+    base = runs[0][1]
+    for tune, got in runs[1:]:
+        for f in base:
+            assert bit_equal(got[f], base[f]), f"{tune}: {f} differs from the automatic shape"
+    if over.get("CLOSURE") == "corey":
+        od = oracle.OracleDomain(keys, st)
+        od.run(6)
+        want = od.fields()
+        for f in od.names:
+            assert bit_equal(base[f], want[f]), f"{f}: max rel {max_rel(base[f], want[f]):.3e} against the oracle"
+
+
 def test_headline_physics_bitwise_at_96():
     """The benchmark physics (three-phase, Corey-2, lognormal K, top
     injection) at 96^3: one and three fused steps equal the oracle bit for bit."""
