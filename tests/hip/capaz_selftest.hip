@@ -55,10 +55,13 @@ __global__ void k_div(uint64_t n, uint64_t seed, int emin, int emax, unsigned lo
     if (local) atomicAdd(bad, local);
 }
 
+// log_cl exists on the device only (the host build of the closures calls pow)
 __global__ void k_log(const double* x, double* y, uint64_t n) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
 #if defined(__HIP_DEVICE_COMPILE__)
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] = capaz::log_cl(x[i]);
+#else
+    (void)x, (void)y, (void)n;
 #endif
 }
 
